@@ -1,0 +1,147 @@
+/*
+ * pvl.h -- C ABI of libpvl_hip.so, the gfx950 implementation of PyVL's hot path:
+ * line-vortex induction (matrix assembly and point-velocity sums), the dense
+ * circulation solve, and the wake shed/convect step.
+ *
+ * Conventions shared by every entry point:
+ *   - Plain pointers and sizes only.  "dev" pointers are device (HBM) buffers that the
+ *     caller owns (allocate them with pvl_malloc or any HIP allocator); "host" pointers
+ *     are ordinary host memory.
+ *   - Vectors are AoS float64 triples, 24 bytes with no padding (the layout of the
+ *     reference's real3_t, src/core/common.h:63-77).
+ *   - Line connectivity is two uint32 per line; surface CSR is uint32 offsets[n+1] plus
+ *     uint32 entries whose bit 31 is the orientation flag and bits 0..30 the line index
+ *     (the reference's geo_id_t / line_t, src/core/common.h:46-56, src/core/mesh.h:10-14).
+ *   - Every function returns an int status (PVL_OK = 0, negative on failure) and sets a
+ *     thread-local message readable with pvl_last_error().
+ *   - All device work is issued on one library stream per process (default priority);
+ *     functions that return device results are asynchronous unless stated otherwise,
+ *     and pvl_synchronize() waits for them.
+ */
+#ifndef PVL_H
+#define PVL_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum pvl_status
+{
+    PVL_OK = 0,
+    PVL_ERR_ARG = -1,      /* invalid argument (maps to ValueError/TypeError in the Python shim) */
+    PVL_ERR_HIP = -2,      /* a HIP runtime call failed */
+    PVL_ERR_NOMEM = -3,    /* device or host allocation failed (MemoryError) */
+    PVL_ERR_SINGULAR = -4, /* exact zero pivot met in an LU factorisation */
+    PVL_ERR_NOTCONV = -5,  /* iterative method did not reach its tolerance */
+    PVL_ERR_NODEV = -6,    /* no usable gfx950 device */
+};
+
+enum pvl_precision
+{
+    PVL_FP64 = 0, /* reference arithmetic: FP64 throughout, reference operation order, no FMA */
+    PVL_FP32 = 1, /* mixed: FP64 geometry, cut-off tests and h^2; FP32 transcendental part */
+};
+
+/* ------------------------------------------------------------------------------------
+ * Runtime
+ * ---------------------------------------------------------------------------------- */
+const char *pvl_last_error(void);
+int pvl_init(int device);                 /* select device, create the library stream */
+int pvl_device_count(int *count);
+int pvl_synchronize(void);
+int pvl_malloc(void **dev_ptr, size_t bytes);
+int pvl_free(void *dev_ptr);
+int pvl_memcpy_h2d(void *dev_dst, const void *host_src, size_t bytes);
+int pvl_memcpy_d2h(void *host_dst, const void *dev_src, size_t bytes);
+int pvl_memcpy_d2d(void *dev_dst, const void *dev_src, size_t bytes);
+int pvl_memset_zero(void *dev_dst, size_t bytes);
+
+/* Kernel timing: when enabled, the main kernel of each entry point is bracketed by HIP
+ * events on the library stream; totals are kept per entry-point name. */
+int pvl_prof_enable(int on);
+int pvl_prof_reset(void);
+int pvl_prof_query(const char *name, double *total_ms, long *count);
+/* Generic event timer on the library stream (for whole-step wall times). */
+int pvl_timer_start(void);
+int pvl_timer_stop(double *ms);
+
+/* ------------------------------------------------------------------------------------
+ * Host topology (replaces src/core/mesh.c:175-247 mesh_from_elements,
+ * src/core/mesh.c:70-173 mesh_dual_from_primal, src/core/mesh.c:249-301 mesh_to_elements,
+ * and src/meshobject.c:780-860 merge).  Same line numbering (first appearance order) and
+ * orientation bits, built with a hash map in O(total points) instead of O(E*L).
+ * ---------------------------------------------------------------------------------- */
+/* lines_out must hold 2*sum(point_counts) uint32, surface_offsets_out n_elements+1,
+ * surface_lines_out sum(point_counts). */
+int pvl_mesh_from_elements(unsigned n_elements, const unsigned *point_counts, const unsigned *flat_points,
+                           unsigned *n_lines_out, uint32_t *lines_out, uint32_t *surface_offsets_out,
+                           uint32_t *surface_lines_out);
+/* Dual: dual_lines_out[2*n_lines] (INVALID_ID = 0x7FFFFFFF where absent),
+ * dual_offsets_out[n_points+1], dual_entries_out[2*n_lines]. */
+int pvl_mesh_dual(unsigned n_points, unsigned n_lines, const uint32_t *lines, unsigned n_surfaces,
+                  const uint32_t *surface_offsets, const uint32_t *surface_lines, uint32_t *dual_lines_out,
+                  uint32_t *dual_offsets_out, uint32_t *dual_entries_out);
+/* Wake ring mesh of python/pyvl/wake_models.py:35-52 built in closed form:
+ * n_strips strips of line_rows rings, nodes 2*(line_rows+1)*n_strips.  Output sizes:
+ * lines_out 2*n_lines with n_lines = n_strips*(3*line_rows+1), offsets n_strips*line_rows+1,
+ * entries 4*n_strips*line_rows. */
+int pvl_wake_topology(unsigned n_strips, unsigned line_rows, unsigned *n_lines_out, uint32_t *lines_out,
+                      uint32_t *surface_offsets_out, uint32_t *surface_lines_out);
+
+/* ------------------------------------------------------------------------------------
+ * Device mesh: the topology uploaded once, plus the line->surface transpose used by the
+ * point-velocity sums.  Replaces the mesh_t owned by cvl.Mesh (src/meshobject.c:25-124).
+ * ---------------------------------------------------------------------------------- */
+typedef struct pvl_mesh pvl_mesh;
+int pvl_mesh_create(pvl_mesh **out, unsigned n_points, unsigned n_lines, const uint32_t *lines, unsigned n_surfaces,
+                    const uint32_t *surface_offsets, const uint32_t *surface_lines);
+int pvl_mesh_destroy(pvl_mesh *mesh);
+int pvl_mesh_counts(const pvl_mesh *mesh, unsigned *n_points, unsigned *n_lines, unsigned *n_surfaces);
+
+/* ------------------------------------------------------------------------------------
+ * Induction (device buffers).
+ * ---------------------------------------------------------------------------------- */
+/* Mesh.induction_matrix3 (src/meshobject.c:384-473 = src/core/flow_solver.c:86-139 then
+ * :174-205): out[n_cpts][n_surfaces] = (sum over surface lines of +-K(line, cpt)) . normal.
+ * Fused: no line buffer. */
+int pvl_induction_matrix3(const pvl_mesh *mesh, double tol, const double *positions_dev, unsigned n_cpts,
+                          const double *cpts_dev, const double *normals_dev, double *out_dev, int precision);
+/* Mesh.induction_matrix (src/meshobject.c:475-556 = flow_solver.c:86-172):
+ * out[n_cpts][n_surfaces][3]. */
+int pvl_induction_matrix(const pvl_mesh *mesh, double tol, const double *positions_dev, unsigned n_cpts,
+                         const double *cpts_dev, double *out_dev, int precision);
+/* Mesh.line_induction_matrix (src/meshobject.c:1232-1286 = flow_solver.c:86-139):
+ * out[n_cpts][n_lines][3]. */
+int pvl_line_induction_matrix(const pvl_mesh *mesh, double tol, const double *positions_dev, unsigned n_cpts,
+                              const double *cpts_dev, double *out_dev, int precision);
+/* Point-velocity sum: the reference materialises induction_matrix and then takes
+ * np.vecdot(ind, circulation[None,:,None], axis=1) (python/pyvl/wake_models.py:125-126,
+ * :170-175, python/pyvl/solver.py:274-280).  Here out[i] (+)= sum_surface Gamma_s * K_s(p_i)
+ * is evaluated as sum over lines of gamma_line * K(line, p_i) with no tensor in HBM.
+ * accumulate != 0 adds into out instead of overwriting. */
+int pvl_point_velocity(const pvl_mesh *mesh, double tol, const double *positions_dev, const double *circulation_dev,
+                       unsigned n_pts, const double *pts_dev, double *out_dev, int accumulate, int precision);
+/* Mesh.surface_normal / Mesh.surface_average_vec3 (src/meshobject.c:956-1062 =
+ * src/core/mesh.c:48-68 and :24-46). */
+int pvl_surface_normal(const pvl_mesh *mesh, const double *positions_dev, double *out_dev);
+int pvl_surface_center(const pvl_mesh *mesh, const double *positions_dev, double *out_dev);
+
+/* Host-buffer convenience forms of the three tensor entry points: stage through device
+ * memory and return when the result is in out_host.  These are what a C caller such as
+ * the reference's src/meshobject.c would call in place of compute_line_induction +
+ * line_induction_to_(normal_)surface_induction. */
+int pvl_induction_matrix3_host(const pvl_mesh *mesh, double tol, const double *positions, unsigned n_cpts,
+                               const double *cpts, const double *normals, double *out, int precision);
+int pvl_induction_matrix_host(const pvl_mesh *mesh, double tol, const double *positions, unsigned n_cpts,
+                              const double *cpts, double *out, int precision);
+int pvl_line_induction_matrix_host(const pvl_mesh *mesh, double tol, const double *positions, unsigned n_cpts,
+                                   const double *cpts, double *out, int precision);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PVL_H */

@@ -1,0 +1,607 @@
+// Line-vortex induction on gfx950: dense matrix assembly, induction tensors and
+// point-velocity sums.  All kernels are compiled with -ffp-contract=off, so every FP64
+// expression below rounds exactly as written (no FMA contraction), matching the
+// reference's x86-64 baseline build.
+#include "pvl_internal.hpp"
+#include "induction_core.hpp"
+
+using namespace pvl;
+
+namespace
+{
+
+constexpr int kWave = 64;
+constexpr int kBlock = 256; // 4 waves
+constexpr int kCptTile = 64; // control points per workgroup (one per lane)
+
+// ---------------------------------------------------------------------------------------
+// Per-line prologue (src/core/flow_solver.c:95-103): end points, |d| and d/|d| with three
+// true divisions.
+__global__ void __launch_bounds__(256) line_geom_kernel(unsigned n_lines, const uint32_t *__restrict__ lines,
+                                                       const double *__restrict__ pos, LineGeom *__restrict__ out)
+{
+    const unsigned i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n_lines)
+        return;
+    const uint32_t p1 = lines[2 * i] & kInvalidId, p2 = lines[2 * i + 1] & kInvalidId;
+    LineGeom g;
+    g.r1x = pos[3 * p1 + 0], g.r1y = pos[3 * p1 + 1], g.r1z = pos[3 * p1 + 2];
+    g.r2x = pos[3 * p2 + 0], g.r2y = pos[3 * p2 + 1], g.r2z = pos[3 * p2 + 2];
+    const double dx = g.r2x - g.r1x, dy = g.r2y - g.r1y, dz = g.r2z - g.r1z;
+    g.len = sqrt(dx * dx + dy * dy + dz * dz);
+    g.ex = dx / g.len;
+    g.ey = dy / g.len;
+    g.ez = dz / g.len;
+    out[i] = g;
+}
+
+__device__ __forceinline__ unsigned wave_id() { return __builtin_amdgcn_readfirstlane(threadIdx.x / kWave); }
+__device__ __forceinline__ unsigned lane_id() { return threadIdx.x % kWave; }
+
+// ---------------------------------------------------------------------------------------
+// Surface-reduced assembly: one lane per control point, the wave walks a run of surfaces
+// (wave-uniform, so surface/line metadata and line geometry come through scalar loads).
+// Each surface accumulates +-K in its CSR order (src/core/flow_solver.c:188-200) and, for
+// NORMAL, dots with the control point's normal (:202).  K of the lines of the previous
+// surface is kept in registers: consecutive surfaces of structured meshes share a line.
+// Results are staged in LDS and written row-contiguous.
+//   NORMAL: out[n_cpts][n_surf]      (induction_matrix3)
+//   VEC:    out[n_cpts][n_surf][3]   (induction_matrix)
+template <bool NORMAL, int PREC, int SURF_PER_WAVE>
+__global__ void __launch_bounds__(kBlock)
+    surface_induction_kernel(unsigned n_cpts, unsigned n_surf, const uint32_t *__restrict__ offsets,
+                             const uint32_t *__restrict__ entries, const LineGeom *__restrict__ geom,
+                             const double *__restrict__ cpts, const double *__restrict__ normals,
+                             double *__restrict__ out, double tol)
+{
+    constexpr int kSurfTile = 4 * SURF_PER_WAVE;
+    constexpr int kComp = NORMAL ? 1 : 3;
+    constexpr int kRow = kSurfTile * kComp + 1; // padded LDS row (doubles)
+    __shared__ double tile[kCptTile * kRow];
+
+    const unsigned w = wave_id(), lane = lane_id();
+    const unsigned cp0 = blockIdx.y * kCptTile;
+    const unsigned s_tile0 = blockIdx.x * kSurfTile;
+    const unsigned cp = cp0 + lane;
+    const bool live = cp < n_cpts;
+    const unsigned cpc = live ? cp : n_cpts - 1;
+    const double px = cpts[3 * cpc + 0], py = cpts[3 * cpc + 1], pz = cpts[3 * cpc + 2];
+    double nx = 0, ny = 0, nz = 0;
+    if (NORMAL)
+        nx = normals[3 * cpc + 0], ny = normals[3 * cpc + 1], nz = normals[3 * cpc + 2];
+    const double tol2 = tol * tol;
+
+    // register cache of the previous surface's first four lines
+    uint32_t oid0 = 0xFFFFFFFFu, oid1 = 0xFFFFFFFFu, oid2 = 0xFFFFFFFFu, oid3 = 0xFFFFFFFFu;
+    double o0x = 0, o0y = 0, o0z = 0, o1x = 0, o1y = 0, o1z = 0, o2x = 0, o2y = 0, o2z = 0, o3x = 0, o3y = 0,
+           o3z = 0;
+
+    for (int j = 0; j < SURF_PER_WAVE; ++j)
+    {
+        const unsigned sl = w * SURF_PER_WAVE + j; // local surface
+        const unsigned s = s_tile0 + sl;
+        double rx = 0.0, ry = 0.0, rz = 0.0;
+        uint32_t nid0 = 0xFFFFFFFFu, nid1 = 0xFFFFFFFFu, nid2 = 0xFFFFFFFFu, nid3 = 0xFFFFFFFFu;
+        double c0x = 0, c0y = 0, c0z = 0, c1x = 0, c1y = 0, c1z = 0, c2x = 0, c2y = 0, c2z = 0, c3x = 0, c3y = 0,
+               c3z = 0;
+        if (s < n_surf)
+        {
+            const uint32_t k0 = offsets[s], k1 = offsets[s + 1];
+            for (uint32_t k = k0; k < k1; ++k)
+            {
+                const uint32_t e = entries[k];
+                const uint32_t l = e & kInvalidId;
+                double kx, ky, kz;
+                if (l == oid0)
+                    kx = o0x, ky = o0y, kz = o0z;
+                else if (l == oid1)
+                    kx = o1x, ky = o1y, kz = o1z;
+                else if (l == oid2)
+                    kx = o2x, ky = o2y, kz = o2z;
+                else if (l == oid3)
+                    kx = o3x, ky = o3y, kz = o3z;
+                else
+                    line_k<PREC>(geom[l], px, py, pz, tol, tol2, kx, ky, kz);
+                const uint32_t slot = k - k0;
+                if (slot == 0)
+                    nid0 = l, c0x = kx, c0y = ky, c0z = kz;
+                else if (slot == 1)
+                    nid1 = l, c1x = kx, c1y = ky, c1z = kz;
+                else if (slot == 2)
+                    nid2 = l, c2x = kx, c2y = ky, c2z = kz;
+                else if (slot == 3)
+                    nid3 = l, c3x = kx, c3y = ky, c3z = kz;
+                if (e & kReversed)
+                {
+                    rx = rx - kx;
+                    ry = ry - ky;
+                    rz = rz - kz;
+                }
+                else
+                {
+                    rx = rx + kx;
+                    ry = ry + ky;
+                    rz = rz + kz;
+                }
+            }
+        }
+        oid0 = nid0, oid1 = nid1, oid2 = nid2, oid3 = nid3;
+        o0x = c0x, o0y = c0y, o0z = c0z, o1x = c1x, o1y = c1y, o1z = c1z;
+        o2x = c2x, o2y = c2y, o2z = c2z, o3x = c3x, o3y = c3y, o3z = c3z;
+        if (NORMAL)
+        {
+            tile[lane * kRow + sl] = rx * nx + ry * ny + rz * nz;
+        }
+        else
+        {
+            tile[lane * kRow + 3 * sl + 0] = rx;
+            tile[lane * kRow + 3 * sl + 1] = ry;
+            tile[lane * kRow + 3 * sl + 2] = rz;
+        }
+    }
+    __syncthreads();
+    // Row-contiguous store: each wave writes 16 control-point rows.
+    const unsigned s_count = (n_surf - s_tile0) < kSurfTile ? (n_surf - s_tile0) : kSurfTile;
+    const unsigned row_len = s_count * kComp;
+    for (unsigned r = w; r < kCptTile; r += 4)
+    {
+        const unsigned cpr = cp0 + r;
+        if (cpr >= n_cpts)
+            break;
+        double *dst = out + ((size_t)cpr * n_surf + s_tile0) * kComp;
+        for (unsigned c = lane; c < row_len; c += kWave)
+            dst[c] = tile[r * kRow + c];
+    }
+}
+
+// Raw line tensor out[n_cpts][n_lines][3] (src/core/flow_solver.c:86-139 with its
+// cpt-major layout :110,136).
+template <int PREC, int LINES_PER_WAVE>
+__global__ void __launch_bounds__(kBlock)
+    line_induction_kernel(unsigned n_cpts, unsigned n_lines, const LineGeom *__restrict__ geom,
+                          const double *__restrict__ cpts, double *__restrict__ out, double tol)
+{
+    constexpr int kLineTile = 4 * LINES_PER_WAVE;
+    constexpr int kRow = kLineTile * 3 + 1;
+    __shared__ double tile[kCptTile * kRow];
+    const unsigned w = wave_id(), lane = lane_id();
+    const unsigned cp0 = blockIdx.y * kCptTile;
+    const unsigned l_tile0 = blockIdx.x * kLineTile;
+    const unsigned cp = cp0 + lane;
+    const unsigned cpc = cp < n_cpts ? cp : n_cpts - 1;
+    const double px = cpts[3 * cpc + 0], py = cpts[3 * cpc + 1], pz = cpts[3 * cpc + 2];
+    const double tol2 = tol * tol;
+    for (int j = 0; j < LINES_PER_WAVE; ++j)
+    {
+        const unsigned ll = w * LINES_PER_WAVE + j;
+        const unsigned l = l_tile0 + ll;
+        double kx = 0, ky = 0, kz = 0;
+        if (l < n_lines)
+            line_k<PREC>(geom[l], px, py, pz, tol, tol2, kx, ky, kz);
+        tile[lane * kRow + 3 * ll + 0] = kx;
+        tile[lane * kRow + 3 * ll + 1] = ky;
+        tile[lane * kRow + 3 * ll + 2] = kz;
+    }
+    __syncthreads();
+    const unsigned l_count = (n_lines - l_tile0) < kLineTile ? (n_lines - l_tile0) : kLineTile;
+    const unsigned row_len = 3 * l_count;
+    for (unsigned r = w; r < kCptTile; r += 4)
+    {
+        const unsigned cpr = cp0 + r;
+        if (cpr >= n_cpts)
+            break;
+        double *dst = out + ((size_t)cpr * n_lines + l_tile0) * 3;
+        for (unsigned c = lane; c < row_len; c += kWave)
+            dst[c] = tile[r * kRow + c];
+    }
+}
+
+// ---------------------------------------------------------------------------------------
+// Point-velocity sums.  Line circulation gamma_l = sum over the surfaces holding line l
+// of +-Gamma_s, in increasing surface order (deterministic).
+__global__ void __launch_bounds__(256) line_circulation_kernel(unsigned n_lines, const uint32_t *__restrict__ loff,
+                                                               const uint32_t *__restrict__ lsurf,
+                                                               const double *__restrict__ circ,
+                                                               double *__restrict__ gamma)
+{
+    const unsigned i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n_lines)
+        return;
+    double g = 0.0;
+    for (uint32_t k = loff[i]; k < loff[i + 1]; ++k)
+    {
+        const uint32_t e = lsurf[k];
+        const double c = circ[e & kInvalidId];
+        g = (e & kReversed) ? g - c : g + c;
+    }
+    gamma[i] = g;
+}
+
+// One lane per point; the workgroup's four waves split a chunk of lines, partial sums
+// are combined in LDS in wave order and written per chunk; a second pass adds the chunks
+// in chunk order.  No atomics: the result does not depend on scheduling.
+template <int PREC>
+__global__ void __launch_bounds__(kBlock)
+    point_velocity_kernel(unsigned n_pts, const double *__restrict__ pts, unsigned n_lines,
+                          const LineGeom *__restrict__ geom, const double *__restrict__ gamma,
+                          unsigned lines_per_chunk, double *__restrict__ partial, double tol)
+{
+    __shared__ double red[4][3][kWave];
+    const unsigned w = wave_id(), lane = lane_id();
+    const unsigned p = blockIdx.x * kWave + lane;
+    const unsigned pc = p < n_pts ? p : n_pts - 1;
+    const double px = pts[3 * pc + 0], py = pts[3 * pc + 1], pz = pts[3 * pc + 2];
+    const double tol2 = tol * tol;
+    const unsigned chunk = blockIdx.y;
+    const unsigned l0 = chunk * lines_per_chunk;
+    const unsigned l1 = (l0 + lines_per_chunk) < n_lines ? (l0 + lines_per_chunk) : n_lines;
+    const unsigned per_wave = (lines_per_chunk + 3) / 4;
+    const unsigned a = l0 + w * per_wave;
+    const unsigned b = (a + per_wave) < l1 ? (a + per_wave) : l1;
+    double vx = 0.0, vy = 0.0, vz = 0.0;
+    for (unsigned l = a; l < b; ++l)
+    {
+        const double g = gamma[l];
+        double kx, ky, kz;
+        line_k<PREC>(geom[l], px, py, pz, tol, tol2, kx, ky, kz);
+        vx = vx + g * kx;
+        vy = vy + g * ky;
+        vz = vz + g * kz;
+    }
+    red[w][0][lane] = vx;
+    red[w][1][lane] = vy;
+    red[w][2][lane] = vz;
+    __syncthreads();
+    if (w == 0 && p < n_pts)
+    {
+        double sx = red[0][0][lane], sy = red[0][1][lane], sz = red[0][2][lane];
+        for (int q = 1; q < 4; ++q)
+        {
+            sx = sx + red[q][0][lane];
+            sy = sy + red[q][1][lane];
+            sz = sz + red[q][2][lane];
+        }
+        double *dst = partial + ((size_t)chunk * n_pts + p) * 3;
+        dst[0] = sx;
+        dst[1] = sy;
+        dst[2] = sz;
+    }
+}
+
+__global__ void __launch_bounds__(256) chunk_reduce_kernel(unsigned n_pts, unsigned n_chunks,
+                                                           const double *__restrict__ partial,
+                                                           double *__restrict__ out, int accumulate)
+{
+    const unsigned i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= 3 * n_pts)
+        return;
+    double s = accumulate ? out[i] : 0.0;
+    for (unsigned c = 0; c < n_chunks; ++c)
+        s = s + partial[(size_t)c * 3 * n_pts + i];
+    out[i] = s;
+}
+
+// ---------------------------------------------------------------------------------------
+// Surface normal and centre (src/core/mesh.c:14-68).
+__global__ void __launch_bounds__(256) surface_geometry_kernel(unsigned n_surf, const uint32_t *__restrict__ offsets,
+                                                               const uint32_t *__restrict__ entries,
+                                                               const uint32_t *__restrict__ lines,
+                                                               const double *__restrict__ pos,
+                                                               double *__restrict__ normal_out,
+                                                               double *__restrict__ center_out)
+{
+    const unsigned s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n_surf)
+        return;
+    const uint32_t k0 = offsets[s], k1 = offsets[s + 1];
+    if (center_out)
+    {
+        // sum of each line's start point in traversal direction, times 1/n
+        double cx = 0.0, cy = 0.0, cz = 0.0;
+        for (uint32_t k = k0; k < k1; ++k)
+        {
+            const uint32_t e = entries[k];
+            const uint32_t l = e & kInvalidId;
+            const uint32_t p = (e & kReversed) ? (lines[2 * l + 1] & kInvalidId) : (lines[2 * l] & kInvalidId);
+            cx = cx + pos[3 * p + 0];
+            cy = cy + pos[3 * p + 1];
+            cz = cz + pos[3 * p + 2];
+        }
+        const double div = 1.0 / (double)(k1 - k0);
+        center_out[3 * s + 0] = cx * div;
+        center_out[3 * s + 1] = cy * div;
+        center_out[3 * s + 2] = cz * div;
+    }
+    if (normal_out)
+    {
+        // line_direction: p1 - p2 for a forward line, p2 - p1 for a reversed one
+        auto dir = [&](uint32_t e, double &x, double &y, double &z) {
+            const uint32_t l = e & kInvalidId;
+            const uint32_t a = lines[2 * l] & kInvalidId, b = lines[2 * l + 1] & kInvalidId;
+            const uint32_t u = (e & kReversed) ? b : a, v = (e & kReversed) ? a : b;
+            x = pos[3 * u + 0] - pos[3 * v + 0];
+            y = pos[3 * u + 1] - pos[3 * v + 1];
+            z = pos[3 * u + 2] - pos[3 * v + 2];
+        };
+        double d1x, d1y, d1z;
+        dir(entries[k0], d1x, d1y, d1z);
+        const double d0x = d1x, d0y = d1y, d0z = d1z;
+        double ox = 0.0, oy = 0.0, oz = 0.0;
+        for (uint32_t k = k0 + 1; k < k1; ++k)
+        {
+            double d2x, d2y, d2z;
+            dir(entries[k], d2x, d2y, d2z);
+            // cross(d2, d1)
+            ox = ox + (d2y * d1z - d2z * d1y);
+            oy = oy + (d2z * d1x - d2x * d1z);
+            oz = oz + (d2x * d1y - d2y * d1x);
+            d1x = d2x, d1y = d2y, d1z = d2z;
+        }
+        // closing term cross(d0, d_last), then unit: times 1/|.|
+        ox = ox + (d0y * d1z - d0z * d1y);
+        oy = oy + (d0z * d1x - d0x * d1z);
+        oz = oz + (d0x * d1y - d0y * d1x);
+        const double inv = 1.0 / sqrt(ox * ox + oy * oy + oz * oz);
+        normal_out[3 * s + 0] = ox * inv;
+        normal_out[3 * s + 1] = oy * inv;
+        normal_out[3 * s + 2] = oz * inv;
+    }
+}
+
+// ---------------------------------------------------------------------------------------
+int compute_line_geom(const pvl_mesh *m, const double *pos, LineGeom **out)
+{
+    LineGeom *g = (LineGeom *)scratch(0, sizeof(LineGeom) * (size_t)(m->n_lines ? m->n_lines : 1));
+    if (!g)
+        return PVL_ERR_NOMEM;
+    if (m->n_lines)
+    {
+        line_geom_kernel<<<(m->n_lines + 255) / 256, 256, 0, stream()>>>(m->n_lines, m->d_lines, pos, g);
+        PVL_HIP(hipGetLastError());
+    }
+    *out = g;
+    return PVL_OK;
+}
+
+int check_common(const pvl_mesh *m, double tol, int precision)
+{
+    if (!m)
+    {
+        set_error("null mesh");
+        return PVL_ERR_ARG;
+    }
+    if (precision != PVL_FP64 && precision != PVL_FP32)
+    {
+        set_error("unknown precision %d", precision);
+        return PVL_ERR_ARG;
+    }
+    (void)tol;
+    return ensure_init();
+}
+
+template <bool NORMAL>
+int launch_surface(const pvl_mesh *m, double tol, const double *pos, unsigned n_cpts, const double *cpts,
+                   const double *normals, double *out, int precision)
+{
+    if (n_cpts == 0 || m->n_surfaces == 0)
+        return PVL_OK;
+    LineGeom *g;
+    if (int st = compute_line_geom(m, pos, &g))
+        return st;
+    constexpr int spw = NORMAL ? 16 : 8;
+    const dim3 grid((m->n_surfaces + 4 * spw - 1) / (4 * spw), (n_cpts + kCptTile - 1) / kCptTile);
+    if (grid.y > 65535u)
+    {
+        set_error("too many control points (%u) for one launch", n_cpts);
+        return PVL_ERR_ARG;
+    }
+    ProfScope prof(NORMAL ? "induction_matrix3" : "induction_matrix");
+    if (precision == PVL_FP64)
+        surface_induction_kernel<NORMAL, PVL_FP64, spw><<<grid, kBlock, 0, stream()>>>(
+            n_cpts, m->n_surfaces, m->d_offsets, m->d_entries, g, cpts, normals, out, tol);
+    else
+        surface_induction_kernel<NORMAL, PVL_FP32, spw><<<grid, kBlock, 0, stream()>>>(
+            n_cpts, m->n_surfaces, m->d_offsets, m->d_entries, g, cpts, normals, out, tol);
+    PVL_HIP(hipGetLastError());
+    return PVL_OK;
+}
+
+} // namespace
+
+extern "C" {
+
+int pvl_induction_matrix3(const pvl_mesh *m, double tol, const double *pos, unsigned n_cpts, const double *cpts,
+                          const double *normals, double *out, int precision)
+{
+    if (int st = check_common(m, tol, precision))
+        return st;
+    return launch_surface<true>(m, tol, pos, n_cpts, cpts, normals, out, precision);
+}
+
+int pvl_induction_matrix(const pvl_mesh *m, double tol, const double *pos, unsigned n_cpts, const double *cpts,
+                         double *out, int precision)
+{
+    if (int st = check_common(m, tol, precision))
+        return st;
+    return launch_surface<false>(m, tol, pos, n_cpts, cpts, nullptr, out, precision);
+}
+
+int pvl_line_induction_matrix(const pvl_mesh *m, double tol, const double *pos, unsigned n_cpts, const double *cpts,
+                              double *out, int precision)
+{
+    if (int st = check_common(m, tol, precision))
+        return st;
+    if (n_cpts == 0 || m->n_lines == 0)
+        return PVL_OK;
+    LineGeom *g;
+    if (int st = compute_line_geom(m, pos, &g))
+        return st;
+    constexpr int lpw = 8;
+    const dim3 grid((m->n_lines + 4 * lpw - 1) / (4 * lpw), (n_cpts + kCptTile - 1) / kCptTile);
+    if (grid.y > 65535u)
+    {
+        set_error("too many control points (%u) for one launch", n_cpts);
+        return PVL_ERR_ARG;
+    }
+    ProfScope prof("line_induction_matrix");
+    if (precision == PVL_FP64)
+        line_induction_kernel<PVL_FP64, lpw><<<grid, kBlock, 0, stream()>>>(n_cpts, m->n_lines, g, cpts, out, tol);
+    else
+        line_induction_kernel<PVL_FP32, lpw><<<grid, kBlock, 0, stream()>>>(n_cpts, m->n_lines, g, cpts, out, tol);
+    PVL_HIP(hipGetLastError());
+    return PVL_OK;
+}
+
+int pvl_point_velocity(const pvl_mesh *m, double tol, const double *pos, const double *circ, unsigned n_pts,
+                       const double *pts, double *out, int accumulate, int precision)
+{
+    if (int st = check_common(m, tol, precision))
+        return st;
+    if (n_pts == 0)
+        return PVL_OK;
+    if (m->n_lines == 0)
+    {
+        if (!accumulate)
+            PVL_HIP(hipMemsetAsync(out, 0, sizeof(double) * 3 * n_pts, stream()));
+        return PVL_OK;
+    }
+    LineGeom *g;
+    if (int st = compute_line_geom(m, pos, &g))
+        return st;
+    double *gamma = (double *)scratch(1, sizeof(double) * m->n_lines);
+    if (!gamma)
+        return PVL_ERR_NOMEM;
+    line_circulation_kernel<<<(m->n_lines + 255) / 256, 256, 0, stream()>>>(m->n_lines, m->d_line_offsets,
+                                                                            m->d_line_surfs, circ, gamma);
+    PVL_HIP(hipGetLastError());
+    // Chunking: aim for >= ~4 workgroups per CU while keeping >= 64 lines per wave.
+    const unsigned pblocks = (n_pts + kWave - 1) / kWave;
+    unsigned n_chunks = (2048 + pblocks - 1) / pblocks;
+    const unsigned max_chunks = (m->n_lines + 255) / 256;
+    if (n_chunks > max_chunks)
+        n_chunks = max_chunks;
+    if (n_chunks < 1)
+        n_chunks = 1;
+    if (n_chunks > 65535u)
+        n_chunks = 65535u;
+    const unsigned lpc = (m->n_lines + n_chunks - 1) / n_chunks;
+    n_chunks = (m->n_lines + lpc - 1) / lpc;
+    double *partial = (double *)scratch(2, sizeof(double) * 3 * (size_t)n_pts * n_chunks);
+    if (!partial)
+        return PVL_ERR_NOMEM;
+    const dim3 grid(pblocks, n_chunks);
+    {
+        ProfScope prof("point_velocity");
+        if (precision == PVL_FP64)
+            point_velocity_kernel<PVL_FP64>
+                <<<grid, kBlock, 0, stream()>>>(n_pts, pts, m->n_lines, g, gamma, lpc, partial, tol);
+        else
+            point_velocity_kernel<PVL_FP32>
+                <<<grid, kBlock, 0, stream()>>>(n_pts, pts, m->n_lines, g, gamma, lpc, partial, tol);
+        PVL_HIP(hipGetLastError());
+    }
+    chunk_reduce_kernel<<<(3 * n_pts + 255) / 256, 256, 0, stream()>>>(n_pts, n_chunks, partial, out, accumulate);
+    PVL_HIP(hipGetLastError());
+    return PVL_OK;
+}
+
+int pvl_surface_normal(const pvl_mesh *m, const double *pos, double *out)
+{
+    if (int st = check_common(m, 0.0, PVL_FP64))
+        return st;
+    if (m->n_surfaces == 0)
+        return PVL_OK;
+    surface_geometry_kernel<<<(m->n_surfaces + 255) / 256, 256, 0, stream()>>>(
+        m->n_surfaces, m->d_offsets, m->d_entries, m->d_lines, pos, out, nullptr);
+    PVL_HIP(hipGetLastError());
+    return PVL_OK;
+}
+
+int pvl_surface_center(const pvl_mesh *m, const double *pos, double *out)
+{
+    if (int st = check_common(m, 0.0, PVL_FP64))
+        return st;
+    if (m->n_surfaces == 0)
+        return PVL_OK;
+    surface_geometry_kernel<<<(m->n_surfaces + 255) / 256, 256, 0, stream()>>>(
+        m->n_surfaces, m->d_offsets, m->d_entries, m->d_lines, pos, nullptr, out);
+    PVL_HIP(hipGetLastError());
+    return PVL_OK;
+}
+
+// Host-buffer forms: stage inputs, run, copy back, synchronise.
+static int stage(const pvl_mesh *m, const double *pos, unsigned n_cpts, const double *cpts, const double *normals,
+                 double **d_pos, double **d_cpts, double **d_norm)
+{
+    const size_t pb = sizeof(double) * 3 * (size_t)m->n_points, cb = sizeof(double) * 3 * (size_t)n_cpts;
+    *d_pos = (double *)scratch(8, pb ? pb : 8);
+    *d_cpts = (double *)scratch(9, cb ? cb : 8);
+    *d_norm = normals ? (double *)scratch(10, cb ? cb : 8) : nullptr;
+    if (!*d_pos || !*d_cpts || (normals && !*d_norm))
+        return PVL_ERR_NOMEM;
+    PVL_HIP(hipMemcpyAsync(*d_pos, pos, pb, hipMemcpyHostToDevice, stream()));
+    PVL_HIP(hipMemcpyAsync(*d_cpts, cpts, cb, hipMemcpyHostToDevice, stream()));
+    if (normals)
+        PVL_HIP(hipMemcpyAsync(*d_norm, normals, cb, hipMemcpyHostToDevice, stream()));
+    return PVL_OK;
+}
+
+static int unstage(double *out, const double *d_out, size_t bytes)
+{
+    PVL_HIP(hipMemcpyAsync(out, d_out, bytes, hipMemcpyDeviceToHost, stream()));
+    PVL_HIP(hipStreamSynchronize(stream()));
+    return PVL_OK;
+}
+
+int pvl_induction_matrix3_host(const pvl_mesh *m, double tol, const double *pos, unsigned n_cpts, const double *cpts,
+                               const double *normals, double *out, int precision)
+{
+    if (int st = check_common(m, tol, precision))
+        return st;
+    double *dp, *dc, *dn;
+    if (int st = stage(m, pos, n_cpts, cpts, normals, &dp, &dc, &dn))
+        return st;
+    const size_t ob = sizeof(double) * (size_t)n_cpts * m->n_surfaces;
+    double *d_out = (double *)scratch(11, ob ? ob : 8);
+    if (!d_out)
+        return PVL_ERR_NOMEM;
+    if (int st = pvl_induction_matrix3(m, tol, dp, n_cpts, dc, dn, d_out, precision))
+        return st;
+    return unstage(out, d_out, ob);
+}
+
+int pvl_induction_matrix_host(const pvl_mesh *m, double tol, const double *pos, unsigned n_cpts, const double *cpts,
+                              double *out, int precision)
+{
+    if (int st = check_common(m, tol, precision))
+        return st;
+    double *dp, *dc, *dn;
+    if (int st = stage(m, pos, n_cpts, cpts, nullptr, &dp, &dc, &dn))
+        return st;
+    const size_t ob = sizeof(double) * 3 * (size_t)n_cpts * m->n_surfaces;
+    double *d_out = (double *)scratch(11, ob ? ob : 8);
+    if (!d_out)
+        return PVL_ERR_NOMEM;
+    if (int st = pvl_induction_matrix(m, tol, dp, n_cpts, dc, d_out, precision))
+        return st;
+    return unstage(out, d_out, ob);
+}
+
+int pvl_line_induction_matrix_host(const pvl_mesh *m, double tol, const double *pos, unsigned n_cpts,
+                                   const double *cpts, double *out, int precision)
+{
+    if (int st = check_common(m, tol, precision))
+        return st;
+    double *dp, *dc, *dn;
+    if (int st = stage(m, pos, n_cpts, cpts, nullptr, &dp, &dc, &dn))
+        return st;
+    const size_t ob = sizeof(double) * 3 * (size_t)n_cpts * m->n_lines;
+    double *d_out = (double *)scratch(11, ob ? ob : 8);
+    if (!d_out)
+        return PVL_ERR_NOMEM;
+    if (int st = pvl_line_induction_matrix(m, tol, dp, n_cpts, dc, d_out, precision))
+        return st;
+    return unstage(out, d_out, ob);
+}
+
+} // extern "C"

@@ -1,0 +1,69 @@
+// Internal declarations shared by the libpvl_hip.so translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <string>
+
+#include "../../include/pvl.h"
+
+namespace pvl
+{
+
+constexpr uint32_t kInvalidId = 0x7FFFFFFFu; // src/core/common.h:54
+constexpr uint32_t kReversed = 0x80000000u;  // src/core/common.h:55
+
+// Error state (thread local) and the library stream.
+void set_error(const char *fmt, ...);
+hipStream_t stream();
+int ensure_init();
+
+// Kernel timing hooks: when profiling is on, brackets the launches between begin/end with
+// HIP events on the library stream and accumulates the elapsed time under `name`.
+struct ProfScope
+{
+    const char *name;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    explicit ProfScope(const char *n);
+    ~ProfScope();
+};
+
+int check_hip(hipError_t e, const char *what);
+#define PVL_HIP(call)                                                                                                  \
+    do                                                                                                                 \
+    {                                                                                                                  \
+        const int _st = ::pvl::check_hip((call), #call);                                                               \
+        if (_st != PVL_OK)                                                                                             \
+            return _st;                                                                                                \
+    } while (0)
+
+// Scratch device buffer reused across calls, grown on demand (one per slot).
+void *scratch(int slot, size_t bytes);
+
+} // namespace pvl
+
+// Line geometry precomputed once per call: start point, end point, unit direction and
+// length, exactly as the reference's per-line prologue (src/core/flow_solver.c:95-103).
+struct alignas(16) LineGeom
+{
+    double r1x, r1y, r1z;
+    double r2x, r2y, r2z;
+    double ex, ey, ez;
+    double len;
+};
+static_assert(sizeof(LineGeom) == 80, "LineGeom layout");
+
+// The device-resident mesh.
+struct pvl_mesh
+{
+    unsigned n_points = 0, n_lines = 0, n_surfaces = 0;
+    unsigned max_surface_lines = 0; // longest surface (entries)
+    uint32_t *d_lines = nullptr;    // [2*n_lines] (bit 31 of each end kept as given)
+    uint32_t *d_offsets = nullptr;  // [n_surfaces+1]
+    uint32_t *d_entries = nullptr;  // [offsets[n_surfaces]]
+    // line -> (surface, sign) transpose, for line circulations
+    uint32_t *d_line_offsets = nullptr; // [n_lines+1]
+    uint32_t *d_line_surfs = nullptr;   // [offsets[n_surfaces]] surface | kReversed
+};

@@ -1,0 +1,302 @@
+// Runtime services of libpvl_hip.so: error reporting, device selection, the library
+// stream, device memory helpers, scratch buffers and event-based kernel timing.
+#include "pvl_internal.hpp"
+
+#include <cstdarg>
+#include <map>
+#include <mutex>
+#include <string>
+#include <vector>
+
+namespace pvl
+{
+
+static thread_local char g_err[1024] = "";
+static hipStream_t g_stream = nullptr;
+static int g_device = -1;
+static std::mutex g_mutex;
+
+void set_error(const char *fmt, ...)
+{
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof(g_err), fmt, ap);
+    va_end(ap);
+}
+
+int check_hip(hipError_t e, const char *what)
+{
+    if (e == hipSuccess)
+        return PVL_OK;
+    set_error("%s failed: %s", what, hipGetErrorString(e));
+    if (e == hipErrorOutOfMemory)
+        return PVL_ERR_NOMEM;
+    return PVL_ERR_HIP;
+}
+
+static int init_locked(int device)
+{
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= 0)
+    {
+        set_error("no HIP device is visible");
+        return PVL_ERR_NODEV;
+    }
+    if (device < 0 || device >= n)
+    {
+        set_error("device %d out of range (%d visible)", device, n);
+        return PVL_ERR_ARG;
+    }
+    hipDeviceProp_t prop;
+    PVL_HIP(hipGetDeviceProperties(&prop, device));
+    if (std::string(prop.gcnArchName).rfind("gfx950", 0) != 0)
+    {
+        set_error("device %d is %s; this library is built for gfx950 only", device, prop.gcnArchName);
+        return PVL_ERR_NODEV;
+    }
+    PVL_HIP(hipSetDevice(device));
+    if (g_stream == nullptr)
+        PVL_HIP(hipStreamCreateWithFlags(&g_stream, hipStreamNonBlocking));
+    g_device = device;
+    return PVL_OK;
+}
+
+int ensure_init()
+{
+    std::lock_guard<std::mutex> lk(g_mutex);
+    if (g_device >= 0)
+    {
+        if (hipSetDevice(g_device) != hipSuccess)
+        {
+            set_error("hipSetDevice(%d) failed", g_device);
+            return PVL_ERR_HIP;
+        }
+        return PVL_OK;
+    }
+    return init_locked(0);
+}
+
+hipStream_t stream() { return g_stream; }
+
+// ---------------------------------------------------------------------------------------
+// Scratch buffers: grown on demand, never shrunk; freed at process end by the runtime.
+static std::vector<std::pair<void *, size_t>> g_scratch(16, {nullptr, 0});
+
+void *scratch(int slot, size_t bytes)
+{
+    auto &s = g_scratch.at(slot);
+    if (s.second >= bytes && s.first)
+        return s.first;
+    if (s.first)
+    {
+        // Make sure no queued kernel still reads the old buffer.
+        hipStreamSynchronize(g_stream);
+        hipFree(s.first);
+        s = {nullptr, 0};
+    }
+    void *p = nullptr;
+    const size_t rounded = (bytes + 0xFFFFF) & ~size_t(0xFFFFF);
+    if (hipMalloc(&p, rounded) != hipSuccess)
+    {
+        set_error("scratch allocation of %zu bytes failed", rounded);
+        return nullptr;
+    }
+    s = {p, rounded};
+    return p;
+}
+
+// ---------------------------------------------------------------------------------------
+// Profiling
+struct ProfEntry
+{
+    double ms = 0;
+    long count = 0;
+};
+static bool g_prof = false;
+static std::map<std::string, ProfEntry> g_prof_tab;
+
+ProfScope::ProfScope(const char *n) : name(n)
+{
+    if (!g_prof)
+        return;
+    hipEventCreate(&ev0);
+    hipEventCreate(&ev1);
+    hipEventRecord(ev0, g_stream);
+}
+
+ProfScope::~ProfScope()
+{
+    if (!ev0)
+        return;
+    hipEventRecord(ev1, g_stream);
+    hipEventSynchronize(ev1);
+    float ms = 0;
+    hipEventElapsedTime(&ms, ev0, ev1);
+    auto &e = g_prof_tab[name];
+    e.ms += ms;
+    e.count += 1;
+    hipEventDestroy(ev0);
+    hipEventDestroy(ev1);
+}
+
+} // namespace pvl
+
+using namespace pvl;
+
+static hipEvent_t g_t0 = nullptr, g_t1 = nullptr;
+
+extern "C" {
+
+const char *pvl_last_error(void) { return g_err; }
+
+int pvl_init(int device)
+{
+    std::lock_guard<std::mutex> lk(g_mutex);
+    if (g_device >= 0)
+    {
+        if (device == g_device)
+            return PVL_OK;
+        set_error("library already bound to device %d", g_device);
+        return PVL_ERR_ARG;
+    }
+    return init_locked(device);
+}
+
+int pvl_device_count(int *count)
+{
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess)
+        n = 0;
+    *count = n;
+    return PVL_OK;
+}
+
+int pvl_synchronize(void)
+{
+    if (int st = ensure_init())
+        return st;
+    PVL_HIP(hipStreamSynchronize(g_stream));
+    return PVL_OK;
+}
+
+int pvl_malloc(void **dev_ptr, size_t bytes)
+{
+    if (int st = ensure_init())
+        return st;
+    *dev_ptr = nullptr;
+    if (bytes == 0)
+        bytes = 256;
+    PVL_HIP(hipMalloc(dev_ptr, bytes));
+    return PVL_OK;
+}
+
+int pvl_free(void *dev_ptr)
+{
+    if (!dev_ptr)
+        return PVL_OK;
+    if (int st = ensure_init())
+        return st;
+    PVL_HIP(hipStreamSynchronize(g_stream));
+    PVL_HIP(hipFree(dev_ptr));
+    return PVL_OK;
+}
+
+int pvl_memcpy_h2d(void *dst, const void *src, size_t bytes)
+{
+    if (int st = ensure_init())
+        return st;
+    if (bytes == 0)
+        return PVL_OK;
+    PVL_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, g_stream));
+    PVL_HIP(hipStreamSynchronize(g_stream));
+    return PVL_OK;
+}
+
+int pvl_memcpy_d2h(void *dst, const void *src, size_t bytes)
+{
+    if (int st = ensure_init())
+        return st;
+    if (bytes == 0)
+        return PVL_OK;
+    PVL_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, g_stream));
+    PVL_HIP(hipStreamSynchronize(g_stream));
+    return PVL_OK;
+}
+
+int pvl_memcpy_d2d(void *dst, const void *src, size_t bytes)
+{
+    if (int st = ensure_init())
+        return st;
+    if (bytes == 0)
+        return PVL_OK;
+    PVL_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, g_stream));
+    return PVL_OK;
+}
+
+int pvl_memset_zero(void *dst, size_t bytes)
+{
+    if (int st = ensure_init())
+        return st;
+    if (bytes == 0)
+        return PVL_OK;
+    PVL_HIP(hipMemsetAsync(dst, 0, bytes, g_stream));
+    return PVL_OK;
+}
+
+int pvl_prof_enable(int on)
+{
+    if (int st = ensure_init())
+        return st;
+    g_prof = on != 0;
+    return PVL_OK;
+}
+
+int pvl_prof_reset(void)
+{
+    g_prof_tab.clear();
+    return PVL_OK;
+}
+
+int pvl_prof_query(const char *name, double *total_ms, long *count)
+{
+    auto it = g_prof_tab.find(name);
+    if (it == g_prof_tab.end())
+    {
+        *total_ms = 0;
+        *count = 0;
+        return PVL_OK;
+    }
+    *total_ms = it->second.ms;
+    *count = it->second.count;
+    return PVL_OK;
+}
+
+int pvl_timer_start(void)
+{
+    if (int st = ensure_init())
+        return st;
+    if (!g_t0)
+    {
+        PVL_HIP(hipEventCreate(&g_t0));
+        PVL_HIP(hipEventCreate(&g_t1));
+    }
+    PVL_HIP(hipEventRecord(g_t0, g_stream));
+    return PVL_OK;
+}
+
+int pvl_timer_stop(double *ms)
+{
+    if (!g_t0)
+    {
+        set_error("pvl_timer_stop without pvl_timer_start");
+        return PVL_ERR_ARG;
+    }
+    PVL_HIP(hipEventRecord(g_t1, g_stream));
+    PVL_HIP(hipEventSynchronize(g_t1));
+    float f = 0;
+    PVL_HIP(hipEventElapsedTime(&f, g_t0, g_t1));
+    *ms = f;
+    return PVL_OK;
+}
+
+} // extern "C"

@@ -1,0 +1,308 @@
+// Mesh topology: construction from element lists, the dual mesh, the closed-form wake
+// ring mesh, and the device-resident mesh object.  Host C++ (built by hipcc only so that
+// the whole library is one toolchain).
+#include "pvl_internal.hpp"
+
+#include <unordered_map>
+#include <vector>
+
+using namespace pvl;
+
+namespace
+{
+
+inline uint64_t pair_key(uint32_t a, uint32_t b)
+{
+    const uint32_t lo = a < b ? a : b, hi = a < b ? b : a;
+    return (uint64_t(hi) << 32) | lo;
+}
+
+} // namespace
+
+extern "C" {
+
+// Same result as the reference's O(E*L) first-match search (src/core/mesh.c:175-247):
+// each unordered endpoint pair forms at most one line, numbered in order of first
+// appearance with p1 = the element's previous point and p2 = the current one; a later
+// element walking it in the opposite direction gets the REVERSED bit.  A degenerate
+// pair (a, a) always matches with orientation 0, as the reference's first test does.
+int pvl_mesh_from_elements(unsigned n_elements, const unsigned *point_counts, const unsigned *flat_points,
+                           unsigned *n_lines_out, uint32_t *lines_out, uint32_t *surface_offsets_out,
+                           uint32_t *surface_lines_out)
+{
+    if (n_elements && (!point_counts || !flat_points))
+    {
+        set_error("mesh_from_elements: null input");
+        return PVL_ERR_ARG;
+    }
+    size_t total = 0;
+    for (unsigned i = 0; i < n_elements; ++i)
+        total += point_counts[i];
+    std::unordered_map<uint64_t, uint32_t> seen;
+    seen.reserve(total);
+    uint32_t n_lines = 0;
+    size_t off = 0;
+    surface_offsets_out[0] = 0;
+    for (unsigned e = 0; e < n_elements; ++e)
+    {
+        const unsigned n = point_counts[e];
+        if (n == 0)
+        {
+            surface_offsets_out[e + 1] = (uint32_t)off;
+            continue;
+        }
+        uint32_t left = flat_points[off + n - 1];
+        for (unsigned k = 0; k < n; ++k)
+        {
+            const uint32_t right = flat_points[off + k];
+            const uint64_t key = pair_key(left, right);
+            auto it = seen.find(key);
+            uint32_t id, orient = 0;
+            if (it == seen.end())
+            {
+                id = n_lines++;
+                seen.emplace(key, id);
+                lines_out[2 * id + 0] = left;
+                lines_out[2 * id + 1] = right;
+            }
+            else
+            {
+                id = it->second;
+                orient = (lines_out[2 * id + 0] == left && lines_out[2 * id + 1] == right) ? 0u : 1u;
+            }
+            surface_lines_out[off + k] = id | (orient << 31);
+            left = right;
+        }
+        off += n;
+        surface_offsets_out[e + 1] = (uint32_t)off;
+    }
+    *n_lines_out = n_lines;
+    return PVL_OK;
+}
+
+// Dual mesh with the reference's rules (src/core/mesh.c:70-173): dual line i joins the
+// first two surfaces (in surface order) that contain primal line i, the one holding it
+// REVERSED in slot p1 and the one holding it forward in slot p2 (only the first
+// occurrence inside a surface counts); dual surface k lists, in line order, every primal
+// line starting at point k (orientation 0) and then ending at it (orientation 1).
+int pvl_mesh_dual(unsigned n_points, unsigned n_lines, const uint32_t *lines, unsigned n_surfaces,
+                  const uint32_t *surface_offsets, const uint32_t *surface_lines, uint32_t *dual_lines_out,
+                  uint32_t *dual_offsets_out, uint32_t *dual_entries_out)
+{
+    std::vector<uint8_t> cnt(n_lines, 0);
+    std::vector<uint32_t> last_surf(n_lines, 0xFFFFFFFFu);
+    for (unsigned i = 0; i < n_lines; ++i)
+    {
+        dual_lines_out[2 * i + 0] = kInvalidId;
+        dual_lines_out[2 * i + 1] = kInvalidId;
+    }
+    for (unsigned s = 0; s < n_surfaces; ++s)
+    {
+        for (uint32_t k = surface_offsets[s]; k < surface_offsets[s + 1]; ++k)
+        {
+            const uint32_t v = surface_lines[k] & kInvalidId;
+            const uint32_t o = surface_lines[k] >> 31;
+            if (v >= n_lines)
+                continue;
+            if (last_surf[v] == s)
+                continue; // only the first occurrence inside a surface
+            last_surf[v] = s;
+            if (cnt[v] >= 2)
+                continue;
+            dual_lines_out[2 * v + (o ? 0 : 1)] = s;
+            cnt[v] += 1;
+        }
+    }
+    // counting sort of line ends by point
+    std::vector<uint32_t> deg(n_points + 1, 0);
+    for (unsigned i = 0; i < n_lines; ++i)
+    {
+        const uint32_t a = lines[2 * i] & kInvalidId, b = lines[2 * i + 1] & kInvalidId;
+        if (a < n_points)
+            deg[a] += 1;
+        if (b < n_points)
+            deg[b] += 1;
+    }
+    dual_offsets_out[0] = 0;
+    for (unsigned p = 0; p < n_points; ++p)
+        dual_offsets_out[p + 1] = dual_offsets_out[p] + deg[p];
+    std::vector<uint32_t> fill(dual_offsets_out, dual_offsets_out + n_points);
+    for (unsigned i = 0; i < n_lines; ++i)
+    {
+        const uint32_t a = lines[2 * i] & kInvalidId, b = lines[2 * i + 1] & kInvalidId;
+        if (a < n_points)
+            dual_entries_out[fill[a]++] = i;
+        if (b < n_points)
+            dual_entries_out[fill[b]++] = i | kReversed;
+    }
+    return PVL_OK;
+}
+
+// Closed form of mesh_from_elements applied to the ring list of
+// python/pyvl/wake_models.py:35-52 (ring r of strip s = [r, r+R+1, r+R+2, r+1] + 2(R+1)s):
+// ring 0 creates four lines, every later ring three, and reuses the previous ring's
+// closing rung reversed.  Line base of strip s is s*(3R+1).
+int pvl_wake_topology(unsigned n_strips, unsigned line_rows, unsigned *n_lines_out, uint32_t *lines_out,
+                      uint32_t *surface_offsets_out, uint32_t *surface_lines_out)
+{
+    if (line_rows == 0)
+    {
+        set_error("wake_topology: line_rows must be positive");
+        return PVL_ERR_ARG;
+    }
+    const uint32_t R = line_rows;
+    const uint32_t per_strip = 3 * R + 1;
+    for (uint32_t s = 0; s < n_strips; ++s)
+    {
+        const uint32_t b = 2 * (R + 1) * s; // first node of the strip
+        const uint32_t B = per_strip * s;   // first line of the strip
+        // ring 0: (e,a), (a,c), (c,d), (d,e)
+        uint32_t *L = lines_out + 2 * B;
+        L[0] = b + 1, L[1] = b;
+        L[2] = b, L[3] = b + R + 1;
+        L[4] = b + R + 1, L[5] = b + R + 2;
+        L[6] = b + R + 2, L[7] = b + 1;
+        for (uint32_t r = 1; r < R; ++r)
+        {
+            uint32_t *Lr = lines_out + 2 * (B + 3 * r + 1);
+            Lr[0] = b + r + 1, Lr[1] = b + r;
+            Lr[2] = b + r + R + 1, Lr[3] = b + r + R + 2;
+            Lr[4] = b + r + R + 2, Lr[5] = b + r + 1;
+        }
+        for (uint32_t r = 0; r < R; ++r)
+        {
+            const uint32_t surf = s * R + r;
+            surface_offsets_out[surf] = 4 * surf;
+            uint32_t *E = surface_lines_out + 4 * surf;
+            if (r == 0)
+            {
+                E[0] = B, E[1] = B + 1, E[2] = B + 2, E[3] = B + 3;
+            }
+            else
+            {
+                E[0] = B + 3 * r + 1;
+                E[1] = (B + 3 * r) | kReversed;
+                E[2] = B + 3 * r + 2;
+                E[3] = B + 3 * r + 3;
+            }
+        }
+    }
+    surface_offsets_out[n_strips * R] = 4 * n_strips * R;
+    *n_lines_out = n_strips * per_strip;
+    return PVL_OK;
+}
+
+int pvl_mesh_create(pvl_mesh **out, unsigned n_points, unsigned n_lines, const uint32_t *lines, unsigned n_surfaces,
+                    const uint32_t *surface_offsets, const uint32_t *surface_lines)
+{
+    *out = nullptr;
+    if (int st = ensure_init())
+        return st;
+    if (surface_offsets[0] != 0)
+    {
+        set_error("surface offsets must start at 0");
+        return PVL_ERR_ARG;
+    }
+    for (unsigned i = 0; i < n_lines; ++i)
+    {
+        const uint32_t a = lines[2 * i] & kInvalidId, b = lines[2 * i + 1] & kInvalidId;
+        if (a >= n_points || b >= n_points)
+        {
+            set_error("line %u references point (%u, %u) but the mesh has %u points", i, a, b, n_points);
+            return PVL_ERR_ARG;
+        }
+    }
+    const uint32_t n_entries = surface_offsets[n_surfaces];
+    unsigned max_len = 0;
+    for (unsigned s = 0; s < n_surfaces; ++s)
+    {
+        if (surface_offsets[s + 1] < surface_offsets[s])
+        {
+            set_error("surface offsets are not monotone at surface %u", s);
+            return PVL_ERR_ARG;
+        }
+        const unsigned len = surface_offsets[s + 1] - surface_offsets[s];
+        if (len > max_len)
+            max_len = len;
+    }
+    for (uint32_t k = 0; k < n_entries; ++k)
+    {
+        if ((surface_lines[k] & kInvalidId) >= n_lines)
+        {
+            set_error("surface entry %u references line %u but the mesh has %u lines", k,
+                      surface_lines[k] & kInvalidId, n_lines);
+            return PVL_ERR_ARG;
+        }
+    }
+    // line -> (surface | sign) transpose, surfaces in increasing order inside a line
+    std::vector<uint32_t> loff(n_lines + 1, 0), lsurf(n_entries);
+    for (uint32_t k = 0; k < n_entries; ++k)
+        loff[(surface_lines[k] & kInvalidId) + 1] += 1;
+    for (unsigned i = 0; i < n_lines; ++i)
+        loff[i + 1] += loff[i];
+    {
+        std::vector<uint32_t> fill(loff.begin(), loff.end() - 1);
+        for (unsigned s = 0; s < n_surfaces; ++s)
+            for (uint32_t k = surface_offsets[s]; k < surface_offsets[s + 1]; ++k)
+            {
+                const uint32_t l = surface_lines[k] & kInvalidId;
+                lsurf[fill[l]++] = s | (surface_lines[k] & kReversed);
+            }
+    }
+    pvl_mesh *m = new pvl_mesh;
+    m->n_points = n_points;
+    m->n_lines = n_lines;
+    m->n_surfaces = n_surfaces;
+    m->max_surface_lines = max_len;
+    auto up = [&](uint32_t **dst, const uint32_t *src, size_t count) -> int {
+        const size_t bytes = (count ? count : 1) * sizeof(uint32_t);
+        if (hipMalloc(dst, bytes) != hipSuccess)
+        {
+            set_error("device allocation of %zu bytes for the mesh failed", bytes);
+            return PVL_ERR_NOMEM;
+        }
+        if (count)
+            return check_hip(hipMemcpy(*dst, src, count * sizeof(uint32_t), hipMemcpyHostToDevice), "hipMemcpy");
+        return PVL_OK;
+    };
+    int st = up(&m->d_lines, lines, 2 * (size_t)n_lines);
+    if (!st)
+        st = up(&m->d_offsets, surface_offsets, n_surfaces + 1);
+    if (!st)
+        st = up(&m->d_entries, surface_lines, n_entries);
+    if (!st)
+        st = up(&m->d_line_offsets, loff.data(), n_lines + 1);
+    if (!st)
+        st = up(&m->d_line_surfs, lsurf.data(), n_entries);
+    if (st)
+    {
+        pvl_mesh_destroy(m);
+        return st;
+    }
+    *out = m;
+    return PVL_OK;
+}
+
+int pvl_mesh_destroy(pvl_mesh *m)
+{
+    if (!m)
+        return PVL_OK;
+    hipStreamSynchronize(stream());
+    hipFree(m->d_lines);
+    hipFree(m->d_offsets);
+    hipFree(m->d_entries);
+    hipFree(m->d_line_offsets);
+    hipFree(m->d_line_surfs);
+    delete m;
+    return PVL_OK;
+}
+
+int pvl_mesh_counts(const pvl_mesh *m, unsigned *n_points, unsigned *n_lines, unsigned *n_surfaces)
+{
+    *n_points = m->n_points;
+    *n_lines = m->n_lines;
+    *n_surfaces = m->n_surfaces;
+    return PVL_OK;
+}
+
+} // extern "C"
