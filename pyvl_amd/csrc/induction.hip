@@ -294,6 +294,19 @@ __global__ void __launch_bounds__(256) surface_geometry_kernel(unsigned n_surf, 
     if (s >= n_surf)
         return;
     const uint32_t k0 = offsets[s], k1 = offsets[s + 1];
+    if (k1 <= k0)
+    {
+        // empty surface: no entries to read (the reference divides 0 by 0 here)
+        const double nan = __builtin_nan("");
+        for (int c = 0; c < 3; ++c)
+        {
+            if (center_out)
+                center_out[3 * s + c] = nan;
+            if (normal_out)
+                normal_out[3 * s + c] = nan;
+        }
+        return;
+    }
     if (center_out)
     {
         // sum of each line's start point in traversal direction, times 1/n
