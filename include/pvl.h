@@ -140,6 +140,37 @@ int pvl_induction_matrix_host(const pvl_mesh *mesh, double tol, const double *po
 int pvl_line_induction_matrix_host(const pvl_mesh *mesh, double tol, const double *positions, unsigned n_cpts,
                                    const double *cpts, double *out, int precision);
 
+/* ------------------------------------------------------------------------------------
+ * Dense solve.  Replaces scipy.linalg.lu_factor / lu_solve as called by run_solver
+ * (python/pyvl/solver.py:223-227; LAPACK dgetrf/dgetrs semantics: P A = L U with partial
+ * pivoting, piv[j] = row interchanged with row j, 0-based).  Matrices are row-major
+ * n x n on the device; right-hand sides are n x nrhs row-major.
+ * ---------------------------------------------------------------------------------- */
+typedef struct pvl_lu pvl_lu;
+/* precision PVL_FP64: factors kept in FP64; PVL_FP32: factors kept in FP32 (MFMA FP32). */
+int pvl_lu_create(pvl_lu **out, int n, int precision);
+int pvl_lu_destroy(pvl_lu *h);
+/* Copy A (FP64, device) into the handle (converting for FP32) and factor it.  Returns
+ * PVL_ERR_SINGULAR with *info_out = first zero pivot (1-based) when U is exactly singular. */
+int pvl_lu_factor(pvl_lu *h, const double *A_dev, int *info_out);
+/* Solve in the handle's precision: B_dev (double* for FP64 handles, float* for FP32) is
+ * overwritten with the solution.  1 <= nrhs <= 32. */
+int pvl_lu_solve(pvl_lu *h, void *B_dev, int nrhs);
+int pvl_lu_pivots(const pvl_lu *h, int *piv_host);
+int pvl_lu_factors(const pvl_lu *h, void *lu_dev_out);
+/* R = B - A X in FP64. */
+int pvl_residual(int n, const double *A_dev, const double *X_dev, const double *B_dev, double *R_dev, int nrhs);
+/* Mixed precision solve: h holds the FP32 factorisation of A (pvl_lu_factor on an FP32
+ * handle); X_dev receives the FP64 solution of A X = B after FP64 iterative refinement,
+ * stopping when every column has max|dx| <= tol * max|x|.  No reference counterpart: the
+ * reference only has the FP64 LAPACK solve. */
+int pvl_solve_refine(pvl_lu *h, const double *A_dev, const double *B_dev, double *X_dev, int nrhs, double tol,
+                     int max_iter, int *iters_out, double *res_out);
+/* Closed-body deflation A[i][j] += c on the body's diagonal block [r0, r1): the rank-one
+ * shift that makes the closed-surface system (A 1 = 0, python/pyvl/solver.py:229-233)
+ * nonsingular without changing its mean-free solution. */
+int pvl_deflate(int n, double *A_dev, int r0, int r1, double c);
+
 #ifdef __cplusplus
 }
 #endif

@@ -87,6 +87,18 @@ SIGNATURES: dict[str, tuple] = {
         C.c_int,
         [C.c_void_p, C.c_double, _hp, C.c_uint, _hp, _hp, C.c_int],
     ),
+    "pvl_lu_create": (C.c_int, [C.POINTER(C.c_void_p), C.c_int, C.c_int]),
+    "pvl_lu_destroy": (C.c_int, [C.c_void_p]),
+    "pvl_lu_factor": (C.c_int, [C.c_void_p, _dp, C.POINTER(C.c_int)]),
+    "pvl_lu_solve": (C.c_int, [C.c_void_p, _dp, C.c_int]),
+    "pvl_lu_pivots": (C.c_int, [C.c_void_p, C.POINTER(C.c_int)]),
+    "pvl_lu_factors": (C.c_int, [C.c_void_p, _dp]),
+    "pvl_residual": (C.c_int, [C.c_int, _dp, _dp, _dp, _dp, C.c_int]),
+    "pvl_solve_refine": (
+        C.c_int,
+        [C.c_void_p, _dp, _dp, _dp, C.c_int, C.c_double, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_double)],
+    ),
+    "pvl_deflate": (C.c_int, [C.c_int, _dp, C.c_int, C.c_int, C.c_double]),
 }
 
 _LIB: C.CDLL | None = None

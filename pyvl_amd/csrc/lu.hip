@@ -1,0 +1,1073 @@
+// Dense LU with partial pivoting on gfx950 (FP64 and FP32), triangular solves, FP64
+// residuals and mixed-precision iterative refinement.
+//
+// The reference solves with scipy.linalg.lu_factor / lu_solve (python/pyvl/solver.py:223-227),
+// i.e. LAPACK dgetrf/dgetrs: P A = L U with the first largest |a| of each column as pivot.
+// Here A is row-major n x n (lda = n): the same factorisation as LAPACK applied to the
+// same matrix, with piv[j] = row swapped with row j at step j (0-based, as scipy returns).
+//
+// Right-looking blocked algorithm, panel width NB = 64:
+//   panel_kernel   cooperative grid; each workgroup holds up to R panel rows in LDS; one
+//                  grid-wide exchange per column carries every group's pivot candidate
+//                  together with the candidate row, so a column costs one barrier.
+//   laswp_kernel   applies the panel's interchanges to the other columns.
+//   trsm_kernel    U12 = L11^-1 A12 (one thread per column, L11 in LDS).
+//   gemm_kernel    A22 -= L21 U12 on MFMA (v_mfma_f64_16x16x4f64 / v_mfma_f32_16x16x4f32).
+// Solves use persistent dataflow kernels over 64-row blocks with the inverses of the
+// diagonal blocks precomputed once per factorisation.
+#include "pvl_internal.hpp"
+
+#include <climits>
+#include <cmath>
+#include <vector>
+
+using namespace pvl;
+
+namespace
+{
+
+constexpr int NB = 64;        // panel / block width
+constexpr int kMaxGroups = 256;
+constexpr int kMaxRhs = 32;   // right-hand sides per solve launch
+constexpr unsigned kSpinLimit = 20000000u; // bounded waits: ~1 s, then the call fails
+
+typedef double double4_t __attribute__((ext_vector_type(4)));
+typedef float float4_t __attribute__((ext_vector_type(4)));
+
+template <typename T> struct Mfma;
+// Accumulator layouts (measured on gfx950): element v of lane l holds
+//   f64 16x16x4: D[4*v + l/16][l%16]      f32 16x16x4: D[4*(l/16) + v][l%16]
+template <> struct Mfma<double>
+{
+    using acc_t = double4_t;
+    static __device__ __forceinline__ int out_row(int lk, int v) { return 4 * v + lk; }
+    static __device__ __forceinline__ acc_t mma(double a, double b, acc_t c)
+    {
+        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+    }
+};
+template <> struct Mfma<float>
+{
+    using acc_t = float4_t;
+    static __device__ __forceinline__ int out_row(int lk, int v) { return 4 * lk + v; }
+    static __device__ __forceinline__ acc_t mma(float a, float b, acc_t c)
+    {
+        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+    }
+};
+
+template <typename T> struct Bits;
+template <> struct Bits<double>
+{
+    using U = unsigned long long;
+};
+template <> struct Bits<float>
+{
+    using U = unsigned int;
+};
+
+// Cache-bypassing loads/stores for the grid-wide exchange buffers.
+template <typename T> __device__ __forceinline__ T xload(const T *p)
+{
+    using U = typename Bits<T>::U;
+    U u = __hip_atomic_load((const U *)p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return __builtin_bit_cast(T, u);
+}
+template <typename T> __device__ __forceinline__ void xstore(T *p, T v)
+{
+    using U = typename Bits<T>::U;
+    __hip_atomic_store((U *)p, __builtin_bit_cast(U, v), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ int xload_i(const int *p)
+{
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void xstore_i(int *p, int v)
+{
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Grid barrier on a monotonically increasing counter; bounded: on timeout sets *err.
+__device__ __forceinline__ void grid_barrier(unsigned *counter, unsigned target, int *err)
+{
+    __threadfence();
+    __syncthreads();
+    if (threadIdx.x == 0)
+    {
+        __hip_atomic_fetch_add(counter, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        unsigned spins = 0;
+        while (__hip_atomic_load(counter, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < target)
+        {
+            __builtin_amdgcn_s_sleep(1);
+            if (++spins > kSpinLimit || __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+            {
+                __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                break;
+            }
+        }
+    }
+    __syncthreads();
+}
+
+__device__ __forceinline__ bool better(double v1, int p1, double v2, int p2)
+{
+    return v1 > v2 || (v1 == v2 && p1 < p2);
+}
+
+// Exchange area for the panel kernel (double buffered by column parity).
+template <typename T> struct PanelX
+{
+    unsigned counter; // grid-barrier arrivals, monotonic over one factorisation
+    int pad[3];
+    double cval[2][kMaxGroups];
+    int cpos[2][kMaxGroups];
+    T crow[2][kMaxGroups][NB];
+    T rowj[2][NB];
+};
+
+// ---------------------------------------------------------------------------------------
+// Panel factorisation of P (m x jb, row stride lda), rows k.. of the matrix.  Group g owns
+// panel positions [g*R, g*R+R).  Per column j: local argmax over positions >= j, publish
+// (|value|, position, row) plus the current row j, grid barrier, every group picks the
+// same winner (largest |a|, first position on ties: LAPACK idamax), swaps, scales by the
+// reciprocal pivot and applies the rank-1 update to its rows.
+template <typename T>
+__global__ void __launch_bounds__(256) panel_kernel(T *__restrict__ P, int lda, int m, int jb, int R, int row_base,
+                                                    int *__restrict__ piv, PanelX<T> *__restrict__ X, unsigned base,
+                                                    int *__restrict__ status)
+{
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    constexpr int LDP = NB + 1;
+    T *L = (T *)smem_raw;
+    __shared__ T prow[NB];
+    __shared__ double red_v[4];
+    __shared__ int red_p[4];
+    __shared__ int s_pos, s_grp;
+
+    const int tid = threadIdx.x, g = blockIdx.x, G = gridDim.x;
+    const int r0 = g * R;
+    const int nr = max(0, min(m, r0 + R) - r0);
+    for (int idx = tid; idx < nr * jb; idx += 256)
+    {
+        const int r = idx / jb, c = idx - r * jb;
+        L[r * LDP + c] = P[(size_t)(r0 + r) * lda + c];
+    }
+    __syncthreads();
+
+    for (int j = 0; j < jb; ++j)
+    {
+        const int par = j & 1;
+        // 1. local candidate
+        double bv = -1.0;
+        int bp = INT_MAX;
+        for (int r = tid; r < nr; r += 256)
+        {
+            const int pos = r0 + r;
+            if (pos >= j)
+            {
+                const double v = fabs((double)L[r * LDP + j]);
+                if (better(v, pos, bv, bp))
+                    bv = v, bp = pos;
+            }
+        }
+        for (int off = 32; off > 0; off >>= 1)
+        {
+            const double ov = __shfl_xor(bv, off);
+            const int op = __shfl_xor(bp, off);
+            if (better(ov, op, bv, bp))
+                bv = ov, bp = op;
+        }
+        if ((tid & 63) == 0)
+            red_v[tid >> 6] = bv, red_p[tid >> 6] = bp;
+        __syncthreads();
+        bv = red_v[0], bp = red_p[0];
+        for (int w = 1; w < 4; ++w)
+            if (better(red_v[w], red_p[w], bv, bp))
+                bv = red_v[w], bp = red_p[w];
+        // 2. publish candidate, its row, and the current row j if owned
+        if (tid == 0)
+        {
+            xstore(&X->cval[par][g], bv);
+            xstore_i(&X->cpos[par][g], bp);
+        }
+        if (bp != INT_MAX)
+            for (int c = tid; c < jb; c += 256)
+                xstore(&X->crow[par][g][c], L[(bp - r0) * LDP + c]);
+        const bool own_j = j >= r0 && j < r0 + nr;
+        if (own_j)
+            for (int c = tid; c < jb; c += 256)
+                xstore(&X->rowj[par][c], L[(j - r0) * LDP + c]);
+        // 3. barrier
+        if (G > 1)
+            grid_barrier(&X->counter, base + (unsigned)G * (unsigned)(j + 1), &status[1]);
+        else
+            __syncthreads();
+        // 4. winner (identical in every group)
+        if (tid < 64)
+        {
+            double v = -1.0;
+            int p = INT_MAX, wg = 0;
+            for (int q = tid; q < G; q += 64)
+            {
+                const double qv = xload(&X->cval[par][q]);
+                const int qp = xload_i(&X->cpos[par][q]);
+                if (better(qv, qp, v, p))
+                    v = qv, p = qp, wg = q;
+            }
+            for (int off = 32; off > 0; off >>= 1)
+            {
+                const double ov = __shfl_xor(v, off);
+                const int op = __shfl_xor(p, off);
+                const int og = __shfl_xor(wg, off);
+                if (better(ov, op, v, p))
+                    v = ov, p = op, wg = og;
+            }
+            if (tid == 0)
+                s_pos = p, s_grp = wg;
+        }
+        __syncthreads();
+        int p = s_pos;
+        const int wg = s_grp;
+        const bool none = (p == INT_MAX); // every candidate NaN: keep row j in place
+        if (none)
+            p = j;
+        for (int c = tid; c < jb; c += 256)
+            prow[c] = none ? xload(&X->rowj[par][c]) : xload(&X->crow[par][wg][c]);
+        __syncthreads();
+        // 5. interchange
+        const bool own_p = p >= r0 && p < r0 + nr;
+        if (p != j)
+        {
+            if (own_j)
+                for (int c = tid; c < jb; c += 256)
+                    L[(j - r0) * LDP + c] = prow[c];
+            if (own_p)
+                for (int c = tid; c < jb; c += 256)
+                    L[(p - r0) * LDP + c] = xload(&X->rowj[par][c]);
+        }
+        if (g == 0 && tid == 0)
+            piv[row_base + j] = row_base + p;
+        __syncthreads();
+        // 6. scale by the reciprocal pivot and update the trailing panel columns
+        const T pv = prow[j];
+        if (pv != T(0))
+        {
+            const T inv = T(1) / pv;
+            for (int r = tid; r < nr; r += 256)
+            {
+                if (r0 + r > j)
+                {
+                    T *row = L + r * LDP;
+                    const T l = row[j] * inv;
+                    row[j] = l;
+                    for (int c = j + 1; c < jb; ++c)
+                        row[c] = fma(-l, prow[c], row[c]);
+                }
+            }
+        }
+        else if (g == 0 && tid == 0)
+        {
+            // first exactly-zero pivot, as LAPACK's info (1-based global column)
+            atomicCAS(&status[0], 0, row_base + j + 1);
+        }
+        __syncthreads();
+    }
+    for (int idx = tid; idx < nr * jb; idx += 256)
+    {
+        const int r = idx / jb, c = idx - r * jb;
+        P[(size_t)(r0 + r) * lda + c] = L[r * LDP + c];
+    }
+}
+
+// Row interchanges of one panel applied to every column outside it.
+template <typename T>
+__global__ void __launch_bounds__(256) laswp_kernel(T *__restrict__ A, int lda, int n, int k, int jb,
+                                                   const int *__restrict__ piv)
+{
+    const int c = blockIdx.x * 256 + threadIdx.x;
+    if (c >= n || (c >= k && c < k + jb))
+        return;
+    for (int i = 0; i < jb; ++i)
+    {
+        const int p = piv[k + i];
+        if (p != k + i)
+        {
+            const T a = A[(size_t)(k + i) * lda + c];
+            A[(size_t)(k + i) * lda + c] = A[(size_t)p * lda + c];
+            A[(size_t)p * lda + c] = a;
+        }
+    }
+}
+
+// U12 = L11^-1 A12 for the columns right of the panel: one workgroup per 64-column tile,
+// the tile and unit-lower L11 in LDS, right-looking sweep (step t finalises row t and
+// updates rows t+1.. of all 64 columns; 256 threads = 64 columns x 4 row groups).
+template <typename T>
+__global__ void __launch_bounds__(256) trsm_kernel(T *__restrict__ A, int lda, int n, int k, int jb)
+{
+    __shared__ T Ls[NB][NB + 1];
+    __shared__ T Xs[NB][NB + 1];
+    const int tid = threadIdx.x;
+    const int c0 = k + jb + blockIdx.x * NB;
+    for (int idx = tid; idx < NB * NB; idx += 256)
+    {
+        const int r = idx / NB, c = idx % NB;
+        Ls[r][c] = (r < jb && c < jb && c < r) ? A[(size_t)(k + r) * lda + k + c] : T(0);
+        Xs[r][c] = (r < jb && c0 + c < n) ? A[(size_t)(k + r) * lda + c0 + c] : T(0);
+    }
+    __syncthreads();
+    const int c = tid & 63, rg = tid >> 6;
+    for (int t = 0; t < jb - 1; ++t)
+    {
+        const T xt = Xs[t][c];
+        for (int i = t + 1 + rg; i < jb; i += 4)
+            Xs[i][c] = fma(-Ls[i][t], xt, Xs[i][c]);
+        __syncthreads();
+    }
+    for (int idx = tid; idx < NB * NB; idx += 256)
+    {
+        const int r = idx / NB, cc = idx % NB;
+        if (r < jb && c0 + cc < n)
+            A[(size_t)(k + r) * lda + c0 + cc] = Xs[r][cc];
+    }
+}
+
+// A22 -= L21 U12 on MFMA.  64x64 output tile per workgroup, 32x32 per wave (2x2 MFMA
+// 16x16 tiles).  Operand lane layout of the 16x16x4 MFMAs: lane l supplies A[l%16][l/16]
+// and B[l/16][l%16]; the accumulator row of element v is Mfma<T>::out_row.
+template <typename T>
+__global__ void __launch_bounds__(256) gemm_kernel(T *__restrict__ A, int lda, int m2, int n2, int k, int jb)
+{
+    constexpr int BM = 64, BN = 64;
+    __shared__ T As[BM][NB + 1];
+    __shared__ T Bs[NB][BN + 1];
+    const int tid = threadIdx.x;
+    const int i0 = blockIdx.y * BM, j0 = blockIdx.x * BN;
+    const int rowA = k + jb, colB = k + jb;
+    const int kpad = (jb + 3) & ~3;
+    for (int idx = tid; idx < BM * NB; idx += 256)
+    {
+        const int r = idx / NB, t = idx % NB;
+        As[r][t] = (i0 + r < m2 && t < jb) ? A[(size_t)(rowA + i0 + r) * lda + k + t] : T(0);
+    }
+    for (int idx = tid; idx < NB * BN; idx += 256)
+    {
+        const int t = idx / BN, c = idx % BN;
+        Bs[t][c] = (t < jb && j0 + c < n2) ? A[(size_t)(k + t) * lda + colB + j0 + c] : T(0);
+    }
+    __syncthreads();
+    using acc_t = typename Mfma<T>::acc_t;
+    const int w = tid >> 6, lane = tid & 63;
+    const int wm = w >> 1, wn = w & 1;
+    const int lr = lane & 15, lk = lane >> 4;
+    acc_t acc[2][2];
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b)
+            acc[a][b] = acc_t{0, 0, 0, 0};
+    for (int kk = 0; kk < kpad; kk += 4)
+    {
+        T af[2], bf[2];
+#pragma unroll
+        for (int a = 0; a < 2; ++a)
+            af[a] = As[32 * wm + 16 * a + lr][kk + lk];
+#pragma unroll
+        for (int b = 0; b < 2; ++b)
+            bf[b] = Bs[kk + lk][32 * wn + 16 * b + lr];
+#pragma unroll
+        for (int a = 0; a < 2; ++a)
+#pragma unroll
+            for (int b = 0; b < 2; ++b)
+                acc[a][b] = Mfma<T>::mma(af[a], bf[b], acc[a][b]);
+    }
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b)
+#pragma unroll
+            for (int v = 0; v < 4; ++v)
+            {
+                const int r = i0 + 32 * wm + 16 * a + Mfma<T>::out_row(lk, v);
+                const int c = j0 + 32 * wn + 16 * b + lr;
+                if (r < m2 && c < n2)
+                {
+                    T *dst = A + (size_t)(rowA + r) * lda + colB + c;
+                    *dst = *dst - acc[a][b][v];
+                }
+            }
+}
+
+// Inverses of the 64x64 diagonal blocks: unit lower part -> dinvL, upper part -> dinvU.
+template <typename T>
+__global__ void __launch_bounds__(256) diag_inverse_kernel(const T *__restrict__ A, int lda, int n,
+                                                           T *__restrict__ dinvL, T *__restrict__ dinvU)
+{
+    __shared__ T S[NB][NB + 1];
+    __shared__ T Xl[NB][NB + 1];
+    __shared__ T Xu[NB][NB + 1];
+    const int b = blockIdx.x, r0 = b * NB, nb = min(NB, n - r0);
+    for (int idx = threadIdx.x; idx < NB * NB; idx += 256)
+    {
+        const int r = idx / NB, c = idx % NB;
+        T v;
+        if (r < nb && c < nb)
+            v = A[(size_t)(r0 + r) * lda + r0 + c];
+        else
+            v = (r == c) ? T(1) : T(0); // identity padding
+        S[r][c] = v;
+    }
+    __syncthreads();
+    const int c = threadIdx.x;
+    if (c < NB)
+    {
+        // L X = e_c, L unit lower
+        for (int r = 0; r < NB; ++r)
+        {
+            T s = (r == c) ? T(1) : T(0);
+            for (int t = 0; t < r; ++t)
+                s = fma(-S[r][t], Xl[t][c], s);
+            Xl[r][c] = s;
+        }
+    }
+    else if (c < 2 * NB)
+    {
+        const int cc = c - NB;
+        // U X = e_cc, U upper
+        for (int r = NB - 1; r >= 0; --r)
+        {
+            T s = (r == cc) ? T(1) : T(0);
+            for (int t = r + 1; t < NB; ++t)
+                s = fma(-S[r][t], Xu[t][cc], s);
+            Xu[r][cc] = s / S[r][r];
+        }
+    }
+    __syncthreads();
+    for (int idx = threadIdx.x; idx < NB * NB; idx += 256)
+    {
+        const int r = idx / NB, cc = idx % NB;
+        dinvL[(size_t)b * NB * NB + idx] = Xl[r][cc];
+        dinvU[(size_t)b * NB * NB + idx] = Xu[r][cc];
+    }
+}
+
+// Gather rows: Y[i] = B[perm[i]] (row-major, nrhs columns).
+template <typename T>
+__global__ void __launch_bounds__(256) permute_rows_kernel(const T *__restrict__ B, T *__restrict__ Y, int n, int nrhs,
+                                                           const int *__restrict__ perm)
+{
+    const int idx = blockIdx.x * 256 + threadIdx.x;
+    if (idx >= n * nrhs)
+        return;
+    const int i = idx / nrhs, q = idx - i * nrhs;
+    Y[idx] = B[(size_t)perm[i] * nrhs + q];
+}
+
+// Persistent block triangular solve on B (n x nrhs, row-major), in place.  LOWER: unit
+// lower L, forward order; else upper U, backward order.  Group i owns block row i (in
+// solve order) and folds in every earlier block's solution as soon as its flag shows the
+// current epoch; then multiplies by the precomputed inverse of its diagonal block.
+template <typename T, bool LOWER>
+__global__ void __launch_bounds__(256) block_trsv_kernel(const T *__restrict__ A, int lda, int n, T *__restrict__ B,
+                                                         int nrhs, const T *__restrict__ dinv,
+                                                         unsigned *__restrict__ flags, unsigned epoch,
+                                                         int *__restrict__ err)
+{
+    __shared__ T Ls[NB][NB + 1];
+    __shared__ T xs[NB][kMaxRhs + 1];
+    __shared__ T acc[NB][kMaxRhs + 1];
+    const int nblk = (n + NB - 1) / NB;
+    const int step = blockIdx.x;
+    const int ib = LOWER ? step : nblk - 1 - step;
+    const int r0 = ib * NB, nr = min(NB, n - r0);
+    const int tid = threadIdx.x;
+    for (int idx = tid; idx < NB * nrhs; idx += 256)
+    {
+        const int r = idx / nrhs, q = idx - r * nrhs;
+        acc[r][q] = (r < nr) ? B[(size_t)(r0 + r) * nrhs + q] : T(0);
+    }
+    for (int s = 0; s < step; ++s)
+    {
+        const int jb = LOWER ? s : nblk - 1 - s;
+        const int c0 = jb * NB, nc = min(NB, n - c0);
+        if (tid == 0)
+        {
+            unsigned spins = 0;
+            while (__hip_atomic_load(&flags[jb], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) != epoch)
+            {
+                __builtin_amdgcn_s_sleep(1);
+                if (++spins > kSpinLimit || __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+                {
+                    __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    break;
+                }
+            }
+        }
+        __syncthreads();
+        for (int idx = tid; idx < NB * NB; idx += 256)
+        {
+            const int r = idx / NB, c = idx % NB;
+            Ls[r][c] = (r < nr && c < nc) ? A[(size_t)(r0 + r) * lda + c0 + c] : T(0);
+        }
+        for (int idx = tid; idx < NB * nrhs; idx += 256)
+        {
+            const int t = idx / nrhs, q = idx - t * nrhs;
+            xs[t][q] = (t < nc) ? xload(&B[(size_t)(c0 + t) * nrhs + q]) : T(0);
+        }
+        __syncthreads();
+        for (int idx = tid; idx < NB * nrhs; idx += 256)
+        {
+            const int r = idx / nrhs, q = idx - r * nrhs;
+            T s2 = acc[r][q];
+            for (int t = 0; t < NB; ++t)
+                s2 = fma(-Ls[r][t], xs[t][q], s2);
+            acc[r][q] = s2;
+        }
+        __syncthreads();
+    }
+    // x = inv(D_ii) acc
+    for (int idx = tid; idx < NB * NB; idx += 256)
+        Ls[idx / NB][idx % NB] = dinv[(size_t)ib * NB * NB + idx];
+    __syncthreads();
+    for (int idx = tid; idx < NB * nrhs; idx += 256)
+    {
+        const int r = idx / nrhs, q = idx - r * nrhs;
+        if (r < nr)
+        {
+            T s2 = T(0);
+            for (int t = 0; t < NB; ++t)
+                s2 = fma(Ls[r][t], acc[t][q], s2);
+            xstore(&B[(size_t)(r0 + r) * nrhs + q], s2);
+        }
+    }
+    __threadfence();
+    __syncthreads();
+    if (tid == 0)
+        __hip_atomic_store(&flags[ib], epoch, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// R = B - A X  (FP64; A n x n row-major; X, B, R n x nrhs).  One wave per row.
+template <typename TA>
+__global__ void __launch_bounds__(256) residual_kernel(const TA *__restrict__ A, int lda, int n,
+                                                       const double *__restrict__ X, const double *__restrict__ B,
+                                                       double *__restrict__ R, int nrhs)
+{
+    const int row = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (row >= n)
+        return;
+    double s[kMaxRhs];
+#pragma unroll
+    for (int q = 0; q < kMaxRhs; ++q)
+        s[q] = 0.0;
+    const TA *a = A + (size_t)row * lda;
+    for (int c = lane; c < n; c += 64)
+    {
+        const double av = (double)a[c];
+        const double *x = X + (size_t)c * nrhs;
+#pragma unroll
+        for (int q = 0; q < kMaxRhs; ++q)
+            if (q < nrhs)
+                s[q] = fma(av, x[q], s[q]);
+    }
+#pragma unroll
+    for (int q = 0; q < kMaxRhs; ++q)
+    {
+        if (q < nrhs)
+        {
+            double v = s[q];
+            for (int off = 32; off > 0; off >>= 1)
+                v += __shfl_xor(v, off);
+            if (lane == 0)
+                R[(size_t)row * nrhs + q] = B[(size_t)row * nrhs + q] - v;
+        }
+    }
+}
+
+__global__ void f64_to_f32_kernel(const double *__restrict__ a, float *__restrict__ b, size_t count)
+{
+    for (size_t i = blockIdx.x * (size_t)256 + threadIdx.x; i < count; i += (size_t)gridDim.x * 256)
+        b[i] = (float)a[i];
+}
+
+// Per-column max-norms of D and X (n x nrhs) -> out[0:nrhs] = |D|inf, out[nrhs:2nrhs] = |X|inf.
+__global__ void __launch_bounds__(256) colmax_kernel(const double *__restrict__ D, const double *__restrict__ X,
+                                                     int n, int nrhs, double *__restrict__ out)
+{
+    const int q = blockIdx.x;
+    double md = 0.0, mx = 0.0;
+    for (int i = threadIdx.x; i < n; i += 256)
+    {
+        md = fmax(md, fabs(D[(size_t)i * nrhs + q]));
+        mx = fmax(mx, fabs(X[(size_t)i * nrhs + q]));
+    }
+    __shared__ double sd[256], sx[256];
+    sd[threadIdx.x] = md;
+    sx[threadIdx.x] = mx;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1)
+    {
+        if (threadIdx.x < s)
+        {
+            sd[threadIdx.x] = fmax(sd[threadIdx.x], sd[threadIdx.x + s]);
+            sx[threadIdx.x] = fmax(sx[threadIdx.x], sx[threadIdx.x + s]);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+    {
+        out[q] = sd[0];
+        out[nrhs + q] = sx[0];
+    }
+}
+
+// X += (double) D32  and D64 = (double) D32
+__global__ void add_f32_kernel(double *__restrict__ X, const float *__restrict__ D, double *__restrict__ D64,
+                               size_t count)
+{
+    for (size_t i = blockIdx.x * (size_t)256 + threadIdx.x; i < count; i += (size_t)gridDim.x * 256)
+    {
+        const double d = (double)D[i];
+        D64[i] = d;
+        X[i] = X[i] + d;
+    }
+}
+
+// ---------------------------------------------------------------------------------------
+// Host side
+struct Workspace
+{
+    void *panelx = nullptr;
+    unsigned *flags = nullptr; // per block, epoch-tagged
+    int *err = nullptr;        // err[0]: first zero pivot (factor), err[1]: a bounded wait timed out
+    unsigned epoch = 0;
+    int nblk_cap = 0;
+};
+Workspace g_ws;
+
+int ensure_ws(int nblk)
+{
+    if (!g_ws.panelx)
+    {
+        PVL_HIP(hipMalloc(&g_ws.panelx, sizeof(PanelX<double>)));
+        PVL_HIP(hipMalloc(&g_ws.err, 2 * sizeof(int)));
+        PVL_HIP(hipMemset(g_ws.err, 0, 2 * sizeof(int)));
+    }
+    if (nblk > g_ws.nblk_cap)
+    {
+        if (g_ws.flags)
+        {
+            PVL_HIP(hipStreamSynchronize(stream()));
+            PVL_HIP(hipFree(g_ws.flags));
+        }
+        PVL_HIP(hipMalloc(&g_ws.flags, sizeof(unsigned) * nblk));
+        PVL_HIP(hipMemset(g_ws.flags, 0, sizeof(unsigned) * nblk));
+        g_ws.nblk_cap = nblk;
+        g_ws.epoch = 0;
+    }
+    return PVL_OK;
+}
+
+template <typename T> int panel_rows_max() { return sizeof(T) == 8 ? 256 : 512; }
+
+template <typename T> int factor_impl(int n, T *A, int *piv, int *info_out)
+{
+    const int nblk = (n + NB - 1) / NB;
+    if (int st = ensure_ws(nblk))
+        return st;
+    static bool attr_set[2] = {false, false};
+    const int rmax = panel_rows_max<T>();
+    const size_t lds = sizeof(T) * (size_t)rmax * (NB + 1);
+    if (!attr_set[sizeof(T) == 8])
+    {
+        PVL_HIP(hipFuncSetAttribute((const void *)panel_kernel<T>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                    (int)lds));
+        attr_set[sizeof(T) == 8] = true;
+    }
+    PanelX<T> *X = (PanelX<T> *)g_ws.panelx;
+    int *status = g_ws.err;
+    PVL_HIP(hipMemsetAsync(X, 0, sizeof(unsigned), stream()));
+    PVL_HIP(hipMemsetAsync(status, 0, 2 * sizeof(int), stream()));
+    unsigned base = 0;
+    for (int k = 0; k < n; k += NB)
+    {
+        const int jb = std::min(NB, n - k);
+        const int m = n - k;
+        // rows per group: as many groups as needed for LDS, at most rmax rows each
+        const int G = (m + rmax - 1) / rmax;
+        const int R = (m + G - 1) / G;
+        if (G > kMaxGroups)
+        {
+            set_error("matrix too tall for the panel kernel (%d rows)", m);
+            return PVL_ERR_ARG;
+        }
+        T *P = A + (size_t)k * n + k;
+        int lda = n, row_base = k;
+        int mm = m, jbb = jb, RR = R;
+        unsigned bb = base;
+        void *args[] = {&P, &lda, &mm, &jbb, &RR, &row_base, &piv, &X, &bb, &status};
+        if (G > 1)
+            base += (unsigned)G * (unsigned)jb;
+        const size_t lds_now = sizeof(T) * (size_t)R * (NB + 1);
+        {
+            ProfScope prof(sizeof(T) == 8 ? "lu_panel_f64" : "lu_panel_f32");
+            PVL_HIP(hipLaunchCooperativeKernel((const void *)panel_kernel<T>, dim3(G), dim3(256), args, lds_now,
+                                               stream()));
+        }
+        // info / err readback is deferred to the end (one sync per factorisation)
+        laswp_kernel<T><<<(n + 255) / 256, 256, 0, stream()>>>(A, n, n, k, jb, piv);
+        PVL_HIP(hipGetLastError());
+        const int n2 = n - k - jb;
+        if (n2 > 0)
+        {
+            trsm_kernel<T><<<(n2 + NB - 1) / NB, 256, 0, stream()>>>(A, n, n, k, jb);
+            PVL_HIP(hipGetLastError());
+            const dim3 grid((n2 + 63) / 64, (n2 + 63) / 64);
+            ProfScope prof(sizeof(T) == 8 ? "lu_gemm_f64" : "lu_gemm_f32");
+            gemm_kernel<T><<<grid, 256, 0, stream()>>>(A, n, n2, n2, k, jb);
+            PVL_HIP(hipGetLastError());
+        }
+    }
+    int st2[2];
+    PVL_HIP(hipMemcpyAsync(st2, status, sizeof(st2), hipMemcpyDeviceToHost, stream()));
+    PVL_HIP(hipStreamSynchronize(stream()));
+    if (st2[1])
+    {
+        set_error("LU panel barrier timed out");
+        return PVL_ERR_HIP;
+    }
+    *info_out = st2[0];
+    return PVL_OK;
+}
+
+} // namespace
+
+// Shared with the solver entry points below and other translation units.
+struct pvl_lu
+{
+    int n = 0;
+    int prec = PVL_FP64;
+    void *lu = nullptr;    // factors (T), owned unless borrowed
+    bool own_lu = false;
+    int *piv = nullptr;    // device, n
+    int *perm = nullptr;   // device, n: row i of P B is row perm[i] of B
+    void *dinvL = nullptr; // nblk * 64 * 64 (T)
+    void *dinvU = nullptr;
+    void *work = nullptr;  // n * kMaxRhs (T) staging for permuted right-hand sides
+};
+
+namespace
+{
+
+template <typename T> int finish_factor(pvl_lu *h)
+{
+    const int n = h->n, nblk = (n + NB - 1) / NB;
+    diag_inverse_kernel<T><<<nblk, 256, 0, stream()>>>((const T *)h->lu, n, n, (T *)h->dinvL, (T *)h->dinvU);
+    PVL_HIP(hipGetLastError());
+    // permutation from the interchange list (LAPACK order), computed on the host
+    std::vector<int> piv(n), perm(n);
+    PVL_HIP(hipMemcpyAsync(piv.data(), h->piv, sizeof(int) * n, hipMemcpyDeviceToHost, stream()));
+    PVL_HIP(hipStreamSynchronize(stream()));
+    for (int i = 0; i < n; ++i)
+        perm[i] = i;
+    for (int i = 0; i < n; ++i)
+        std::swap(perm[i], perm[piv[i]]);
+    PVL_HIP(hipMemcpyAsync(h->perm, perm.data(), sizeof(int) * n, hipMemcpyHostToDevice, stream()));
+    PVL_HIP(hipStreamSynchronize(stream()));
+    return PVL_OK;
+}
+
+template <typename T> int solve_impl(pvl_lu *h, T *B, int nrhs)
+{
+    const int n = h->n, nblk = (n + NB - 1) / NB;
+    if (int st = ensure_ws(nblk))
+        return st;
+    T *work = (T *)h->work;
+    for (int q0 = 0; q0 < nrhs; q0 += kMaxRhs)
+    {
+        const int nq = std::min(kMaxRhs, nrhs - q0);
+        // The solve kernels work on a contiguous n x nq block: gather (and permute) into work.
+        if (nq == nrhs)
+        {
+            permute_rows_kernel<T><<<(n * nq + 255) / 256, 256, 0, stream()>>>(B, work, n, nq, h->perm);
+        }
+        else
+        {
+            set_error("more than %d right-hand sides per call are not supported", kMaxRhs);
+            return PVL_ERR_ARG;
+        }
+        PVL_HIP(hipGetLastError());
+        int *err = g_ws.err + 1;
+        PVL_HIP(hipMemsetAsync(err, 0, sizeof(int), stream()));
+        for (int pass = 0; pass < 2; ++pass)
+        {
+            const unsigned epoch = ++g_ws.epoch;
+            const T *Ap = (const T *)h->lu;
+            int lda = n, nn = n, nqq = nq;
+            T *W = work;
+            const T *dinv = pass == 0 ? (const T *)h->dinvL : (const T *)h->dinvU;
+            unsigned *flags = g_ws.flags;
+            unsigned ep = epoch;
+            void *args[] = {(void *)&Ap, &lda, &nn, &W, &nqq, (void *)&dinv, &flags, &ep, &err};
+            const void *fn = pass == 0 ? (const void *)block_trsv_kernel<T, true> : (const void *)block_trsv_kernel<T, false>;
+            ProfScope prof(sizeof(T) == 8 ? "lu_trsv_f64" : "lu_trsv_f32");
+            PVL_HIP(hipLaunchCooperativeKernel(fn, dim3(nblk), dim3(256), args, 0, stream()));
+        }
+        PVL_HIP(hipMemcpyAsync(B, work, sizeof(T) * (size_t)n * nq, hipMemcpyDeviceToDevice, stream()));
+        int e = 0;
+        PVL_HIP(hipMemcpyAsync(&e, err, sizeof(int), hipMemcpyDeviceToHost, stream()));
+        PVL_HIP(hipStreamSynchronize(stream()));
+        if (e)
+        {
+            set_error("triangular solve wait timed out");
+            return PVL_ERR_HIP;
+        }
+    }
+    return PVL_OK;
+}
+
+int alloc_handle(pvl_lu *h, int n, int prec)
+{
+    const size_t es = prec == PVL_FP64 ? 8 : 4;
+    const int nblk = (n + NB - 1) / NB;
+    PVL_HIP(hipMalloc(&h->piv, sizeof(int) * n));
+    PVL_HIP(hipMalloc(&h->perm, sizeof(int) * n));
+    PVL_HIP(hipMalloc(&h->dinvL, es * (size_t)nblk * NB * NB));
+    PVL_HIP(hipMalloc(&h->dinvU, es * (size_t)nblk * NB * NB));
+    PVL_HIP(hipMalloc(&h->work, es * (size_t)n * kMaxRhs));
+    return PVL_OK;
+}
+
+} // namespace
+
+extern "C" {
+
+int pvl_lu_create(pvl_lu **out, int n, int precision)
+{
+    *out = nullptr;
+    if (int st = ensure_init())
+        return st;
+    if (n <= 0 || (precision != PVL_FP64 && precision != PVL_FP32))
+    {
+        set_error("lu_create: bad n=%d or precision=%d", n, precision);
+        return PVL_ERR_ARG;
+    }
+    if ((n + NB - 1) / NB > 2 * kMaxGroups * 2)
+    {
+        set_error("lu_create: n=%d exceeds the supported size", n);
+        return PVL_ERR_ARG;
+    }
+    pvl_lu *h = new pvl_lu;
+    h->n = n;
+    h->prec = precision;
+    const size_t es = precision == PVL_FP64 ? 8 : 4;
+    int st = check_hip(hipMalloc(&h->lu, es * (size_t)n * n), "hipMalloc(lu)");
+    h->own_lu = true;
+    if (!st)
+        st = alloc_handle(h, n, precision);
+    if (st)
+    {
+        pvl_lu_destroy(h);
+        return st;
+    }
+    *out = h;
+    return PVL_OK;
+}
+
+int pvl_lu_destroy(pvl_lu *h)
+{
+    if (!h)
+        return PVL_OK;
+    hipStreamSynchronize(stream());
+    if (h->own_lu)
+        hipFree(h->lu);
+    hipFree(h->piv);
+    hipFree(h->perm);
+    hipFree(h->dinvL);
+    hipFree(h->dinvU);
+    hipFree(h->work);
+    delete h;
+    return PVL_OK;
+}
+
+// Factor: copies A (n x n row-major, FP64 on the device) into the handle's storage in the
+// handle's precision and factors it.  info_out = first zero pivot (1-based) or 0.
+int pvl_lu_factor(pvl_lu *h, const double *A_dev, int *info_out)
+{
+    if (int st = ensure_init())
+        return st;
+    const int n = h->n;
+    const size_t cnt = (size_t)n * n;
+    int info = 0, st;
+    if (h->prec == PVL_FP64)
+    {
+        PVL_HIP(hipMemcpyAsync(h->lu, A_dev, sizeof(double) * cnt, hipMemcpyDeviceToDevice, stream()));
+        st = factor_impl<double>(n, (double *)h->lu, h->piv, &info);
+        if (!st)
+            st = finish_factor<double>(h);
+    }
+    else
+    {
+        f64_to_f32_kernel<<<2048, 256, 0, stream()>>>(A_dev, (float *)h->lu, cnt);
+        PVL_HIP(hipGetLastError());
+        st = factor_impl<float>(n, (float *)h->lu, h->piv, &info);
+        if (!st)
+            st = finish_factor<float>(h);
+    }
+    if (info_out)
+        *info_out = info;
+    if (!st && info)
+    {
+        set_error("exactly singular: U(%d,%d) is zero", info - 1, info - 1);
+        return PVL_ERR_SINGULAR;
+    }
+    return st;
+}
+
+// Solve A X = B in the handle's precision; B (n x nrhs row-major, same precision as the
+// handle) is overwritten with X.
+int pvl_lu_solve(pvl_lu *h, void *B_dev, int nrhs)
+{
+    if (int st = ensure_init())
+        return st;
+    if (nrhs <= 0 || nrhs > kMaxRhs)
+    {
+        set_error("lu_solve: nrhs=%d out of range 1..%d", nrhs, kMaxRhs);
+        return PVL_ERR_ARG;
+    }
+    if (h->prec == PVL_FP64)
+        return solve_impl<double>(h, (double *)B_dev, nrhs);
+    return solve_impl<float>(h, (float *)B_dev, nrhs);
+}
+
+int pvl_lu_pivots(const pvl_lu *h, int *piv_host)
+{
+    PVL_HIP(hipMemcpyAsync(piv_host, h->piv, sizeof(int) * h->n, hipMemcpyDeviceToHost, stream()));
+    PVL_HIP(hipStreamSynchronize(stream()));
+    return PVL_OK;
+}
+
+int pvl_lu_factors(const pvl_lu *h, void *lu_dev_out)
+{
+    const size_t es = h->prec == PVL_FP64 ? 8 : 4;
+    PVL_HIP(hipMemcpyAsync(lu_dev_out, h->lu, es * (size_t)h->n * h->n, hipMemcpyDeviceToDevice, stream()));
+    return PVL_OK;
+}
+
+// R = B - A X in FP64 (A FP64, n x n; X, B, R n x nrhs).
+int pvl_residual(int n, const double *A_dev, const double *X_dev, const double *B_dev, double *R_dev, int nrhs)
+{
+    if (int st = ensure_init())
+        return st;
+    if (nrhs <= 0 || nrhs > kMaxRhs)
+    {
+        set_error("residual: nrhs=%d out of range", nrhs);
+        return PVL_ERR_ARG;
+    }
+    ProfScope prof("residual_f64");
+    residual_kernel<double><<<(n + 3) / 4, 256, 0, stream()>>>(A_dev, n, n, X_dev, B_dev, R_dev, nrhs);
+    PVL_HIP(hipGetLastError());
+    return PVL_OK;
+}
+
+// Mixed precision: FP32 LU of A (handle h, precision FP32, already factored from A_dev),
+// then FP64 iterative refinement x <- x + solve32(b - A x) until every column's
+// max|dx| <= tol * max|x| (or max_iter).  X_dev (n x nrhs FP64) receives the solution.
+// iters_out: refinement steps taken; res_out[nrhs]: final max|dx|/max|x| per column.
+int pvl_solve_refine(pvl_lu *h, const double *A_dev, const double *B_dev, double *X_dev, int nrhs, double tol,
+                     int max_iter, int *iters_out, double *res_out)
+{
+    if (int st = ensure_init())
+        return st;
+    if (h->prec != PVL_FP32)
+    {
+        set_error("solve_refine needs an FP32 factorisation");
+        return PVL_ERR_ARG;
+    }
+    const int n = h->n;
+    if (nrhs <= 0 || nrhs > kMaxRhs)
+    {
+        set_error("solve_refine: nrhs=%d out of range 1..%d", nrhs, kMaxRhs);
+        return PVL_ERR_ARG;
+    }
+    const size_t cnt = (size_t)n * nrhs;
+    double *R = (double *)scratch(3, sizeof(double) * cnt);
+    float *D32 = (float *)scratch(4, sizeof(float) * cnt);
+    double *D64 = (double *)scratch(5, sizeof(double) * cnt);
+    double *norms = (double *)scratch(6, sizeof(double) * 2 * nrhs);
+    if (!R || !D32 || !D64 || !norms)
+        return PVL_ERR_NOMEM;
+    PVL_HIP(hipMemsetAsync(X_dev, 0, sizeof(double) * cnt, stream()));
+    std::vector<double> hn(2 * nrhs);
+    int it = 0;
+    for (; it < max_iter; ++it)
+    {
+        if (int st = pvl_residual(n, A_dev, X_dev, B_dev, R, nrhs))
+            return st;
+        f64_to_f32_kernel<<<512, 256, 0, stream()>>>(R, D32, cnt);
+        PVL_HIP(hipGetLastError());
+        if (int st = solve_impl<float>(h, D32, nrhs))
+            return st;
+        add_f32_kernel<<<512, 256, 0, stream()>>>(X_dev, D32, D64, cnt);
+        PVL_HIP(hipGetLastError());
+        colmax_kernel<<<nrhs, 256, 0, stream()>>>(D64, X_dev, n, nrhs, norms);
+        PVL_HIP(hipGetLastError());
+        PVL_HIP(hipMemcpyAsync(hn.data(), norms, sizeof(double) * 2 * nrhs, hipMemcpyDeviceToHost, stream()));
+        PVL_HIP(hipStreamSynchronize(stream()));
+        bool done = true;
+        for (int q = 0; q < nrhs; ++q)
+        {
+            const double r = hn[nrhs + q] > 0 ? hn[q] / hn[nrhs + q] : hn[q];
+            if (res_out)
+                res_out[q] = r;
+            if (!(r <= tol))
+                done = false;
+        }
+        if (done)
+        {
+            ++it;
+            break;
+        }
+    }
+    if (iters_out)
+        *iters_out = it;
+    for (int q = 0; q < nrhs; ++q)
+    {
+        const double r = hn[nrhs + q] > 0 ? hn[q] / hn[nrhs + q] : hn[q];
+        if (!(r <= tol))
+        {
+            set_error("iterative refinement stalled: column %d at %.3e after %d steps", q, r, it);
+            return PVL_ERR_NOTCONV;
+        }
+    }
+    return PVL_OK;
+}
+
+// Closed-body deflation: A[i][j] += c for i, j in [r0, r1) (rank-one shift c 1 1^T on the
+// body's block; removes the uniform-circulation null direction of a closed surface).
+__global__ void deflate_kernel(double *A, int n, int r0, int r1, double c)
+{
+    const int len = r1 - r0;
+    for (size_t idx = blockIdx.x * (size_t)256 + threadIdx.x; idx < (size_t)len * len;
+         idx += (size_t)gridDim.x * 256)
+    {
+        const int i = r0 + (int)(idx / len), j = r0 + (int)(idx % len);
+        A[(size_t)i * n + j] += c;
+    }
+}
+
+int pvl_deflate(int n, double *A_dev, int r0, int r1, double c)
+{
+    if (int st = ensure_init())
+        return st;
+    if (r0 < 0 || r1 > n || r0 >= r1)
+    {
+        set_error("deflate: bad range [%d, %d) for n=%d", r0, r1, n);
+        return PVL_ERR_ARG;
+    }
+    deflate_kernel<<<2048, 256, 0, stream()>>>(A_dev, n, r0, r1, c);
+    PVL_HIP(hipGetLastError());
+    return PVL_OK;
+}
+
+} // extern "C"

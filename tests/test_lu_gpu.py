@@ -1,0 +1,149 @@
+"""GPU LU against the reference's solver, scipy.linalg.lu_factor / lu_solve (LAPACK
+dgetrf/dgetrs; python/pyvl/solver.py:223-227).
+
+Pivot sequences must equal LAPACK's (first largest |a| per column); factors and solutions
+agree to 1e-12 relative (different summation order inside the trailing updates).  The
+mixed-precision path (FP32 LU + FP64 refinement) must reach 1e-10 relative to the FP64
+solution.
+"""
+
+import numpy as np
+import pytest
+import scipy.linalg as sla
+
+from oracle import oracle as O
+from pyvl_amd import DeviceArray
+from pyvl_amd import geometry as G
+from pyvl_amd import linalg as LA
+from pyvl_amd.cvl import Mesh
+
+pytestmark = pytest.mark.gpu
+
+
+def _rel(a, b):
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+@pytest.mark.parametrize("n", [1, 5, 63, 64, 65, 130, 300, 1000])
+def test_factor_matches_lapack(n):
+    rng = np.random.default_rng(n)
+    a = rng.normal(size=(n, n))
+    lu_ref, piv_ref = sla.lu_factor(a)
+    h = LA.LU(n).factor(a)
+    assert np.array_equal(h.pivots(), piv_ref)
+    lu = h.factors()
+    assert _rel(lu, lu_ref) < 1e-12
+
+
+def test_ties_pick_first_row():
+    """idamax semantics: equal |a| -> the first row."""
+    a = np.array([[1.0, 2.0, 0.5, 1.0], [3.0, 1.0, 1.0, 0.0], [-3.0, 0.0, 2.0, 1.0], [2.0, 1.0, 0.0, 4.0]])
+    _, piv_ref = sla.lu_factor(a)
+    assert np.array_equal(LA.LU(4).factor(a).pivots(), piv_ref)
+
+
+@pytest.mark.parametrize("n,nrhs", [(7, 1), (64, 3), (257, 16), (1000, 32), (1500, 1)])
+def test_solve_matches_lapack(n, nrhs):
+    rng = np.random.default_rng(7 + n)
+    a = rng.normal(size=(n, n)) + n ** 0.5 * np.eye(n)
+    b = rng.normal(size=(n, nrhs)) if nrhs > 1 else rng.normal(size=n)
+    x_ref = sla.lu_solve(sla.lu_factor(a), b)
+    x = LA.LU(n).factor(a).solve(b)
+    assert _rel(x, x_ref) < 1e-12
+
+
+def test_scipy_call_site_dropin():
+    """lu_factor(A, overwrite_a=True) then lu_solve(decomp, b, overwrite_b=True), as
+    python/pyvl/solver.py:223-227 calls them."""
+    rng = np.random.default_rng(3)
+    a = rng.normal(size=(200, 200))
+    b = rng.normal(size=200)
+    x_ref = sla.lu_solve(sla.lu_factor(a.copy()), b.copy())
+    a2, b2 = a.copy(), b.copy()
+    decomp = LA.lu_factor(a2, overwrite_a=True)
+    x = LA.lu_solve(decomp, b2, overwrite_b=True)
+    assert x is b2
+    assert _rel(x, x_ref) < 1e-12
+    lu_ref, piv_ref = sla.lu_factor(a)
+    assert np.array_equal(decomp[1], piv_ref)
+    assert _rel(decomp[0], lu_ref) < 1e-12
+    # a plain (lu, piv) tuple (e.g. from scipy) is accepted too
+    x2 = LA.lu_solve((lu_ref, piv_ref), b)
+    assert _rel(x2, x_ref) < 1e-10
+
+
+def test_singular_reports_info():
+    a = np.array([[1.0, 2.0, 3.0], [2.0, 4.0, 6.0], [1.0, 0.0, 1.0]])
+    with pytest.warns(RuntimeWarning):
+        h = LA.LU(3).factor(a)
+    assert h.info > 0
+
+
+@pytest.mark.parametrize("n,nrhs", [(300, 1), (1000, 4), (2100, 16)])
+def test_fp32_refinement_reaches_fp64(n, nrhs):
+    rng = np.random.default_rng(11 + n)
+    a = rng.normal(size=(n, n)) + 2 * n ** 0.5 * np.eye(n)
+    b = rng.normal(size=(n, nrhs))
+    x_ref = sla.lu_solve(sla.lu_factor(a), b)
+    da, db = DeviceArray.from_numpy(a), DeviceArray.from_numpy(b)
+    x, iters, _, ratios = LA.solve_refine(da, db, tol=1e-13)
+    xh = x.numpy()
+    assert iters <= 10
+    assert np.all(ratios <= 1e-13)
+    for q in range(nrhs):
+        assert _rel(xh[:, q], x_ref[:, q]) < 1e-10
+    # FP32 factors alone are only ~1e-6 accurate
+    lu32 = LA.LU(n, LA.FP32).factor(da)
+    x32 = lu32.solve(b.astype(np.float32))
+    assert 1e-9 < _rel(x32.astype(np.float64), x_ref) < 1e-3
+
+
+def test_residual_kernel():
+    rng = np.random.default_rng(1)
+    a = rng.normal(size=(333, 333))
+    x = rng.normal(size=(333, 5))
+    b = rng.normal(size=(333, 5))
+    r = LA.residual(DeviceArray.from_numpy(a), DeviceArray.from_numpy(x), DeviceArray.from_numpy(b)).numpy()
+    np.testing.assert_allclose(r, b - a @ x, rtol=1e-12, atol=1e-12)
+
+
+def test_flat_wing_system_solve():
+    """C1: steady flat wing, b = n . -(V_inf); GPU LU vs LAPACK on the same matrix."""
+    pos, el, _ = G.flat_wing(16, 4, 8.0, 1.0)
+    m = Mesh(pos.shape[0], el)
+    n = m.surface_normal(pos)
+    c = m.surface_average_vec3(pos)
+    a = m.induction_matrix3(1e-6, pos, c, n)
+    alpha = np.radians(5.0)
+    vinf = np.array([np.cos(alpha), 0.0, np.sin(alpha)])
+    b = np.sum(n * -vinf[None, :], axis=1)
+    x_ref = sla.lu_solve(sla.lu_factor(a), b)
+    x = LA.lu_solve(LA.lu_factor(a), b)
+    assert _rel(x, x_ref) < 1e-13
+    np.testing.assert_allclose(x, -b / (2 * np.pi), rtol=1e-12)
+
+
+def test_closed_body_deflated_solve():
+    """Closed sphere: A 1 ~ 0.  Deflated (A + c 1 1^T) x = b, then mean removal
+    (python/pyvl/solver.py:229-233); compare with LAPACK on the same deflated matrix, and
+    check the mean-free answer does not depend on c."""
+    pos, el = G.uv_sphere(32, 16, sigma=1e-3, seed=0)
+    m = Mesh(pos.shape[0], el)
+    t = O.mesh_from_elements(pos.shape[0], el)
+    n = O.surface_normal(t, pos)
+    c = O.surface_center(t, pos)
+    a = O.induction_matrix3(t, pos, c, n, 1e-6)
+    b = np.sum(n * -np.array([1.0, 0.0, 0.0])[None, :], axis=1)
+    sols = []
+    for shift in (np.mean(np.abs(np.diag(a))), 1.0):
+        da = DeviceArray.from_numpy(a)
+        LA.deflate(da, 0, a.shape[0], shift)
+        ad = da.numpy()
+        np.testing.assert_array_equal(ad, a + shift)
+        x = LA.LU(a.shape[0]).factor(da).solve(b)
+        x -= x.mean()
+        x_ref = sla.lu_solve(sla.lu_factor(a + shift), b)
+        x_ref -= x_ref.mean()
+        assert _rel(x, x_ref) < 1e-12
+        sols.append(x)
+    assert _rel(sols[0], sols[1]) < 1e-10
