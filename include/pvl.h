@@ -171,6 +171,43 @@ int pvl_solve_refine(pvl_lu *h, const double *A_dev, const double *B_dev, double
  * nonsingular without changing its mean-free solution. */
 int pvl_deflate(int n, double *A_dev, int r0, int r1, double c);
 
+/* ------------------------------------------------------------------------------------
+ * Unsteady line wake (python/pyvl/wake_models.py WakeModelLineExplicitUnsteady), resident
+ * in HBM.  Coincident nodes of neighbouring strips are stored once; the reference layout is
+ * available through pvl_wake_get_state / pvl_wake_set_state.
+ * ---------------------------------------------------------------------------------- */
+typedef struct pvl_wake pvl_wake;
+/* WakeModelLineExplicitUnsteady.__init__ (:54-81): bordering_nodes and adjacent_surfaces
+ * are (n_shed, 2) uint32 (INVALID_ID allowed in adjacent_surfaces). */
+int pvl_wake_create(pvl_wake **out, unsigned n_shed, unsigned line_rows, const uint32_t *bordering_nodes,
+                    const uint32_t *adjacent_surfaces, double vortex_tol, double time);
+int pvl_wake_destroy(pvl_wake *w);
+int pvl_wake_info(const pvl_wake *w, unsigned *n_te, unsigned *n_lines, unsigned *step_count, double *current_time);
+/* update (:83-157) with a uniform free stream vinf[3].  mode 0: the reference's convection
+ * (free stream + bound-mesh induction); mode 1: free wake (also the wake's own induction,
+ * i.e. get_velocity of :159-175). */
+int pvl_wake_update(pvl_wake *w, double time, const pvl_mesh *bound, const double *bound_pos_dev,
+                    const double *circulation_dev, const double *vinf, int mode, int precision);
+/* apply_corrections (:177-223) on a row-major n_cpts x n_surf matrix and n_cpts rhs. */
+int pvl_wake_apply_corrections(pvl_wake *w, unsigned n_cpts, const double *cpts_dev, const double *normals_dev,
+                               double *mat_dev, unsigned n_surf, double *rhs_dev, int precision);
+/* get_velocity (:159-175): out (n_pts x 3) (+)= wake-induced velocity. */
+int pvl_wake_get_velocity(pvl_wake *w, unsigned n_pts, const double *pts_dev, double *out_dev, int accumulate,
+                          int precision);
+/* Reference layout: positions (n_shed, 2, R+1, 3), circulation (n_shed, R), host buffers. */
+int pvl_wake_get_state(pvl_wake *w, double *positions_host, double *circulation_host);
+int pvl_wake_set_state(pvl_wake *w, const double *positions_host, const double *circulation_host, unsigned step_count,
+                       double current_time);
+
+/* ------------------------------------------------------------------------------------
+ * Step helpers of run_solver (python/pyvl/solver.py:199-206, :229-233).
+ * ---------------------------------------------------------------------------------- */
+/* out_i = n_i . -(vinf - cpvel_i); cpvel_dev may be NULL (fixed body). */
+int pvl_rhs_normal_velocity(unsigned n, const double *normals_dev, const double *cpvel_dev, const double *vinf,
+                            double *out_dev);
+/* x[r0:r1] -= mean(x[r0:r1]) (closed-body circulation gauge). */
+int pvl_remove_mean(double *x_dev, int r0, int r1);
+
 #ifdef __cplusplus
 }
 #endif

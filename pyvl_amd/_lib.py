@@ -99,6 +99,25 @@ SIGNATURES: dict[str, tuple] = {
         [C.c_void_p, _dp, _dp, _dp, C.c_int, C.c_double, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_double)],
     ),
     "pvl_deflate": (C.c_int, [C.c_int, _dp, C.c_int, C.c_int, C.c_double]),
+    "pvl_wake_create": (
+        C.c_int,
+        [C.POINTER(C.c_void_p), C.c_uint, C.c_uint, _u32p, _u32p, C.c_double, C.c_double],
+    ),
+    "pvl_wake_destroy": (C.c_int, [C.c_void_p]),
+    "pvl_wake_info": (C.c_int, [C.c_void_p, _uintp, _uintp, _uintp, C.POINTER(C.c_double)]),
+    "pvl_wake_update": (
+        C.c_int,
+        [C.c_void_p, C.c_double, C.c_void_p, _dp, _dp, C.POINTER(C.c_double), C.c_int, C.c_int],
+    ),
+    "pvl_wake_apply_corrections": (
+        C.c_int,
+        [C.c_void_p, C.c_uint, _dp, _dp, _dp, C.c_uint, _dp, C.c_int],
+    ),
+    "pvl_wake_get_velocity": (C.c_int, [C.c_void_p, C.c_uint, _dp, _dp, C.c_int, C.c_int]),
+    "pvl_wake_get_state": (C.c_int, [C.c_void_p, _hp, _hp]),
+    "pvl_wake_set_state": (C.c_int, [C.c_void_p, _hp, _hp, C.c_uint, C.c_double]),
+    "pvl_rhs_normal_velocity": (C.c_int, [C.c_uint, _dp, _dp, C.POINTER(C.c_double), _dp]),
+    "pvl_remove_mean": (C.c_int, [_dp, C.c_int, C.c_int]),
 }
 
 _LIB: C.CDLL | None = None

@@ -362,18 +362,23 @@ __global__ void __launch_bounds__(256) surface_geometry_kernel(unsigned n_surf, 
 }
 
 // ---------------------------------------------------------------------------------------
-int compute_line_geom(const pvl_mesh *m, const double *pos, LineGeom **out)
+int compute_line_geom(const pvl_mesh *m, const double *pos, LineGeom **out, unsigned n_lines, int slot = 0)
 {
-    LineGeom *g = (LineGeom *)scratch(0, sizeof(LineGeom) * (size_t)(m->n_lines ? m->n_lines : 1));
+    LineGeom *g = (LineGeom *)scratch(slot, sizeof(LineGeom) * (size_t)(n_lines ? n_lines : 1));
     if (!g)
         return PVL_ERR_NOMEM;
-    if (m->n_lines)
+    if (n_lines)
     {
-        line_geom_kernel<<<(m->n_lines + 255) / 256, 256, 0, stream()>>>(m->n_lines, m->d_lines, pos, g);
+        line_geom_kernel<<<(n_lines + 255) / 256, 256, 0, stream()>>>(n_lines, m->d_lines, pos, g);
         PVL_HIP(hipGetLastError());
     }
     *out = g;
     return PVL_OK;
+}
+
+int compute_line_geom(const pvl_mesh *m, const double *pos, LineGeom **out)
+{
+    return compute_line_geom(m, pos, out, m->n_lines, 0);
 }
 
 int check_common(const pvl_mesh *m, double tol, int precision)
@@ -420,6 +425,67 @@ int launch_surface(const pvl_mesh *m, double tol, const double *pos, unsigned n_
 }
 
 } // namespace
+
+namespace pvl
+{
+
+// Point-velocity sum over the first n_active lines of the mesh (all lines for the public
+// entry point; the wake passes the prefix of lines its filled rows use -- the other lines
+// carry zero circulation).
+int point_velocity_impl(const pvl_mesh *m, double tol, const double *pos, const double *circ, unsigned n_pts,
+                        const double *pts, double *out, int accumulate, int precision, unsigned n_active)
+{
+    if (n_pts == 0)
+        return PVL_OK;
+    if (n_active > m->n_lines)
+        n_active = m->n_lines;
+    if (n_active == 0)
+    {
+        if (!accumulate)
+            PVL_HIP(hipMemsetAsync(out, 0, sizeof(double) * 3 * n_pts, stream()));
+        return PVL_OK;
+    }
+    LineGeom *g;
+    if (int st = compute_line_geom(m, pos, &g, n_active, 0))
+        return st;
+    double *gamma = (double *)scratch(1, sizeof(double) * n_active);
+    if (!gamma)
+        return PVL_ERR_NOMEM;
+    line_circulation_kernel<<<(n_active + 255) / 256, 256, 0, stream()>>>(n_active, m->d_line_offsets,
+                                                                            m->d_line_surfs, circ, gamma);
+    PVL_HIP(hipGetLastError());
+    // Chunking: aim for >= ~4 workgroups per CU while keeping >= 64 lines per wave.
+    const unsigned pblocks = (n_pts + kWave - 1) / kWave;
+    unsigned n_chunks = (2048 + pblocks - 1) / pblocks;
+    const unsigned max_chunks = (n_active + 255) / 256;
+    if (n_chunks > max_chunks)
+        n_chunks = max_chunks;
+    if (n_chunks < 1)
+        n_chunks = 1;
+    if (n_chunks > 65535u)
+        n_chunks = 65535u;
+    const unsigned lpc = (n_active + n_chunks - 1) / n_chunks;
+    n_chunks = (n_active + lpc - 1) / lpc;
+    double *partial = (double *)scratch(2, sizeof(double) * 3 * (size_t)n_pts * n_chunks);
+    if (!partial)
+        return PVL_ERR_NOMEM;
+    const dim3 grid(pblocks, n_chunks);
+    {
+        ProfScope prof("point_velocity");
+        if (precision == PVL_FP64)
+            point_velocity_kernel<PVL_FP64>
+                <<<grid, kBlock, 0, stream()>>>(n_pts, pts, n_active, g, gamma, lpc, partial, tol);
+        else
+            point_velocity_kernel<PVL_FP32>
+                <<<grid, kBlock, 0, stream()>>>(n_pts, pts, n_active, g, gamma, lpc, partial, tol);
+        PVL_HIP(hipGetLastError());
+    }
+    chunk_reduce_kernel<<<(3 * n_pts + 255) / 256, 256, 0, stream()>>>(n_pts, n_chunks, partial, out, accumulate);
+    PVL_HIP(hipGetLastError());
+    return PVL_OK;
+}
+
+} // namespace pvl
 
 extern "C" {
 
@@ -470,52 +536,7 @@ int pvl_point_velocity(const pvl_mesh *m, double tol, const double *pos, const d
 {
     if (int st = check_common(m, tol, precision))
         return st;
-    if (n_pts == 0)
-        return PVL_OK;
-    if (m->n_lines == 0)
-    {
-        if (!accumulate)
-            PVL_HIP(hipMemsetAsync(out, 0, sizeof(double) * 3 * n_pts, stream()));
-        return PVL_OK;
-    }
-    LineGeom *g;
-    if (int st = compute_line_geom(m, pos, &g))
-        return st;
-    double *gamma = (double *)scratch(1, sizeof(double) * m->n_lines);
-    if (!gamma)
-        return PVL_ERR_NOMEM;
-    line_circulation_kernel<<<(m->n_lines + 255) / 256, 256, 0, stream()>>>(m->n_lines, m->d_line_offsets,
-                                                                            m->d_line_surfs, circ, gamma);
-    PVL_HIP(hipGetLastError());
-    // Chunking: aim for >= ~4 workgroups per CU while keeping >= 64 lines per wave.
-    const unsigned pblocks = (n_pts + kWave - 1) / kWave;
-    unsigned n_chunks = (2048 + pblocks - 1) / pblocks;
-    const unsigned max_chunks = (m->n_lines + 255) / 256;
-    if (n_chunks > max_chunks)
-        n_chunks = max_chunks;
-    if (n_chunks < 1)
-        n_chunks = 1;
-    if (n_chunks > 65535u)
-        n_chunks = 65535u;
-    const unsigned lpc = (m->n_lines + n_chunks - 1) / n_chunks;
-    n_chunks = (m->n_lines + lpc - 1) / lpc;
-    double *partial = (double *)scratch(2, sizeof(double) * 3 * (size_t)n_pts * n_chunks);
-    if (!partial)
-        return PVL_ERR_NOMEM;
-    const dim3 grid(pblocks, n_chunks);
-    {
-        ProfScope prof("point_velocity");
-        if (precision == PVL_FP64)
-            point_velocity_kernel<PVL_FP64>
-                <<<grid, kBlock, 0, stream()>>>(n_pts, pts, m->n_lines, g, gamma, lpc, partial, tol);
-        else
-            point_velocity_kernel<PVL_FP32>
-                <<<grid, kBlock, 0, stream()>>>(n_pts, pts, m->n_lines, g, gamma, lpc, partial, tol);
-        PVL_HIP(hipGetLastError());
-    }
-    chunk_reduce_kernel<<<(3 * n_pts + 255) / 256, 256, 0, stream()>>>(n_pts, n_chunks, partial, out, accumulate);
-    PVL_HIP(hipGetLastError());
-    return PVL_OK;
+    return pvl::point_velocity_impl(m, tol, pos, circ, n_pts, pts, out, accumulate, precision, m->n_lines);
 }
 
 int pvl_surface_normal(const pvl_mesh *m, const double *pos, double *out)

@@ -44,6 +44,14 @@ void *scratch(int slot, size_t bytes);
 
 } // namespace pvl
 
+struct pvl_mesh;
+namespace pvl
+{
+// Point-velocity sum restricted to the first n_active lines of the mesh.
+int point_velocity_impl(const pvl_mesh *m, double tol, const double *pos, const double *circ, unsigned n_pts,
+                        const double *pts, double *out, int accumulate, int precision, unsigned n_active);
+} // namespace pvl
+
 // Line geometry precomputed once per call: start point, end point, unit direction and
 // length, exactly as the reference's per-line prologue (src/core/flow_solver.c:95-103).
 struct alignas(16) LineGeom
