@@ -1,0 +1,401 @@
+#!/usr/bin/env python3
+"""Benchmark of the line-vortex wake step and the 16k-panel solve on one MI355X.
+
+Headline (one JSON line on stdout): wake steps/s of the unsteady free-wake workload --
+rectangular flat wing, 64 x 16 = 1,024 bound panels (span 8, chord 1), heave 0.1 chord at
+reduced frequency k = 0.3 (omega = 0.6), dt = c/(16 U), one wake row shed per step into a
+400-row wake (25,600 wake panels; 26,065 distinct wake nodes, 51,664 distinct segments),
+free stream 1 m/s along x, vortex cut-off 1e-6, FP64 throughout.  A step is the reference's
+run_solver iteration (python/pyvl/solver.py:175-239): kinematics, rhs, system matrix,
+wake corrections, LU solve and wake update, with free-wake convection (free stream + bound
++ wake self-induction at every wake node).  The wake is grown to its full 400 rows first;
+`--warmup` further full-wake steps are untimed and `--steps` are timed.
+
+Also reported: the same workload with FP32 induction angles, the reference's own
+convection semantics, and the 16k-panel closed-body solve (ellipsoid 4:1:0.5, 16 right-hand
+sides): plain FP64 LU against FP32 LU on MFMA + FP64 iterative refinement, with the error
+of both against LAPACK FP64 (scipy, as the reference calls it) on the host.
+
+CPU baseline: the oracle's C restatement (OpenMP, all host cores) of the same free-wake
+step, on a bounded sample (one full-wake step started from the GPU's wake state).
+"""
+
+from __future__ import annotations
+
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+import numpy as np  # noqa: E402
+
+PEAK_FP64_TFLOPS = 78.6  # MI355X FP64 (vector and MFMA)
+PEAK_HBM_GBS = 8000.0
+FLOP_PER_PAIR = 87.0  # induction pair (src/core/flow_solver.c:114-135, atan2 counted as 20)
+FLOP_WEIGHT = 6.0  # Gamma weighting of a point-velocity sum
+
+C3 = dict(n_span=64, n_chord=16, span=8.0, chord=1.0, h0=0.1, omega=0.6, dt=1.0 / 16.0, rows=400, tol=1e-6)
+
+
+def log(msg: str) -> None:
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def prof_get(lib, name: str):
+    ms, cnt, work = C.c_double(), C.c_long(), C.c_double()
+    lib.pvl_prof_query(name.encode(), C.byref(ms), C.byref(cnt))
+    lib.pvl_prof_work(name.encode(), C.byref(work))
+    return ms.value, cnt.value, work.value
+
+
+def run_c3(mode: str, prec: int, steps: int, warmup: int, profile: bool):
+    from pyvl_amd import _lib
+    from pyvl_amd.solver import FlowConditionsUniform, HeaveFrame, make_wing_wake_solver
+
+    lib = _lib.load()
+    fc = FlowConditionsUniform(1.0, 0.0, 0.0)
+    frame = HeaveFrame(C3["h0"] * C3["chord"], C3["omega"])
+    sol = make_wing_wake_solver(C3["n_span"], C3["n_chord"], C3["span"], C3["chord"], frame, fc, C3["rows"],
+                                C3["dt"], vortex_limit=C3["tol"], mode=mode, induction_precision=prec)
+    t0 = time.perf_counter()
+    grow = sol.run(C3["rows"])
+    grow_s = time.perf_counter() - t0
+    log(f"{mode}/{'fp64' if prec == 0 else 'fp32'}: wake grown to {C3['rows']} rows in {grow_s:.2f} s "
+        f"(last 50 growth steps {np.mean(grow[-50:]) * 1e3:.2f} ms/step)")
+    sol.run(warmup)
+    if profile:
+        lib.pvl_prof_reset()
+        lib.pvl_prof_enable(1)
+    _lib.call("pvl_synchronize")
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        sol.step()
+    _lib.call("pvl_synchronize")
+    timed = time.perf_counter() - t0
+    prof = {}
+    if profile:
+        for name in ("wake_self_velocity", "wake_bound_velocity", "wake_explicit", "induction_matrix3",
+                     "lu_panel_f64", "lu_gemm_f64", "lu_trsv_f64"):
+            prof[name] = prof_get(lib, name)
+        lib.pvl_prof_enable(0)
+    log(f"{mode}/{'fp64' if prec == 0 else 'fp32'}: {steps} full-wake steps in {timed:.3f} s "
+        f"-> {steps / timed:.1f} steps/s")
+    return dict(sol=sol, timed_s=timed, steps_per_s=steps / timed, ms_per_step=1e3 * timed / steps,
+                grow_400_s=grow_s, prof=prof)
+
+
+def roofline_from(prof) -> dict:
+    ms, cnt, pairs = prof["wake_self_velocity"]
+    flops = pairs * (FLOP_PER_PAIR + FLOP_WEIGHT)
+    avg_ms = ms / max(cnt, 1)
+    achieved = flops / (ms * 1e-3) / 1e12 if ms > 0 else 0.0
+    return {
+        "bound": "mfma",
+        "achieved": round(achieved, 3),
+        "peak": PEAK_FP64_TFLOPS,
+        "unit": "TFLOP/s",
+        "frac": round(achieved / PEAK_FP64_TFLOPS, 4),
+        "traffic": None,
+        "kernel": "point_velocity_kernel<FP64, wake-self> (free-wake self-induction)",
+        "note": "FP64 vector-ALU bound (MI355X FP64 vector peak = FP64 MFMA peak = 78.6 TFLOP/s); "
+                f"algorithmic {FLOP_PER_PAIR + FLOP_WEIGHT:.0f} FLOP per (segment, point) pair, "
+                f"{pairs / max(cnt, 1):.4g} pairs per launch, avg {avg_ms:.3f} ms over {cnt} launches "
+                "(HIP events on the library stream, timed region only)",
+    }
+
+
+def cpu_free_wake_step(gpu_run) -> dict:
+    """One full-wake free-wake step on the host, oracle C restatement (reference layout)."""
+    import scipy.linalg as la
+
+    from oracle import oracle as O
+    from oracle.wake_oracle import OracleWake
+
+    sol = gpu_run["sol"]
+    w = sol.wake
+    pos0 = sol.p0
+    el_counts, flat = sol.mesh.to_element_connectivity()
+    offs = np.concatenate([[0], np.cumsum(el_counts)])
+    elements = [flat[offs[i]:offs[i + 1]].tolist() for i in range(len(el_counts))]
+    bound = O.mesh_from_elements(pos0.shape[0], elements)
+    ow = OracleWake(w.bordering_nodes, w.shedding_lines, w.adjacent_surfaces, C3["tol"], 0.0, C3["rows"])
+    wp, wc = w.state()
+    ow.wake_positions[...] = wp
+    ow.circulation[...] = wc
+    ow.step_count = w.step_count
+    ow.current_time = w.current_time
+    it = sol.iteration
+    vinf = np.array([1.0, 0.0, 0.0])
+    t0 = time.perf_counter()
+    # one run_solver iteration with free-wake convection
+    t = it * sol.dt
+    pos = sol.frame.positions(pos0, t)
+    vel = sol.frame.velocities(pos0, t)
+    cps = O.surface_center(bound, pos)
+    cpv = O.surface_center(bound, vel)
+    nrm = O.surface_normal(bound, pos)
+    b = np.vecdot(nrm, -(np.full_like(cps, vinf) - cpv), axis=1)
+    a = O.induction_matrix3(bound, pos, cps, nrm, C3["tol"])
+    ow.apply_corrections(cps, nrm, a, b)
+    gamma = la.lu_solve(la.lu_factor(a), b)
+    cr = min(ow.step_count, ow.line_rows)
+    old = ow.wake_positions[:, :, :cr, :].copy()
+    flat_pts = np.ascontiguousarray(old.reshape(-1, 3))
+    v = np.full_like(flat_pts, vinf)
+    v += O.point_velocity(bound, pos, gamma, flat_pts, C3["tol"])
+    v += O.point_velocity(ow.wake_mesh, ow.wake_positions.reshape(-1, 3), ow.circulation.reshape(-1), flat_pts,
+                          C3["tol"])
+    ow.wake_positions[:, :, 1:cr + 1, :] = old + v.reshape(old.shape) * sol.dt
+    elapsed = time.perf_counter() - t0
+    return {
+        "value": round(1.0 / elapsed, 4),
+        "unit": "steps/s",
+        "cores": os.cpu_count(),
+        "kind": "port",
+        "sample": f"1 full-wake free-wake step (step {it + 1}, 25,600 wake panels, reference layout: "
+                  f"{flat_pts.shape[0]} wake nodes x {ow.wake_mesh.n_lines} wake lines) from the GPU's wake "
+                  f"state; oracle C restatement, gcc -O3 no FMA, OpenMP {os.cpu_count()} threads; "
+                  f"LU = scipy LAPACK; {elapsed:.2f} s",
+    }
+
+
+def cpu_reference_step(gpu_ref_run, n_steps: int) -> dict:
+    """Reference-semantics steps on the host with the reference's own C core (oracle/_ref)."""
+    import scipy.linalg as la
+
+    from oracle import oracle as O
+    from oracle.wake_oracle import OracleWake
+
+    if O.load_ref() is None:
+        return {"skipped": "oracle/_ref not built"}
+    sol = gpu_ref_run["sol"]
+    w = sol.wake
+    pos0 = sol.p0
+    el_counts, flat = sol.mesh.to_element_connectivity()
+    offs = np.concatenate([[0], np.cumsum(el_counts)])
+    elements = [flat[offs[i]:offs[i + 1]].tolist() for i in range(len(el_counts))]
+    bound = O.mesh_from_elements(pos0.shape[0], elements)
+    ow = OracleWake(w.bordering_nodes, w.shedding_lines, w.adjacent_surfaces, C3["tol"], 0.0, C3["rows"])
+    wp, wc = w.state()
+    ow.wake_positions[...] = wp
+    ow.circulation[...] = wc
+    ow.step_count = w.step_count
+    ow.current_time = w.current_time
+    vinf = np.array([1.0, 0.0, 0.0])
+    it0 = sol.iteration
+    t0 = time.perf_counter()
+    for k in range(n_steps):
+        it = it0 + k
+        t = it * sol.dt
+        pos = sol.frame.positions(pos0, t)
+        vel = sol.frame.velocities(pos0, t)
+        cps = O.ref_surface_center(bound, pos)
+        cpv = O.ref_surface_center(bound, vel)
+        nrm = O.ref_surface_normal(bound, pos)
+        b = np.vecdot(nrm, -(np.full_like(cps, vinf) - cpv), axis=1)
+        a = O.ref_induction_matrix3(bound, pos, cps, nrm, C3["tol"])
+        # wake corrections with the reference's own line-buffer path
+        n = ow.shedding_lines.size
+        ind = O.ref_induction_matrix3(ow.wake_mesh, ow.wake_positions.reshape(-1, 3), cps, nrm, C3["tol"])
+        ind = ind.reshape(cps.shape[0], n, ow.line_rows)
+        for i_line in range(n):
+            corr = ind[:, i_line, 0]
+            s1, s0 = ow.adjacent_surfaces[i_line, 1], ow.adjacent_surfaces[i_line, 0]
+            if s1 != 0x7FFFFFFF:
+                a[:, s1] -= corr
+            if s0 != 0x7FFFFFFF:
+                a[:, s0] += corr
+        b += np.vecdot(ind[:, :, 1:].reshape(cps.shape[0], -1), ow.circulation[:, 1:].reshape(-1)[None, :], axis=1)
+        gamma = la.lu_solve(la.lu_factor(a), b)
+        dt = t - ow.current_time
+        ow.current_time = t
+        cr = min(ow.step_count, ow.line_rows)
+        for i in reversed(range(cr)):
+            old = ow.wake_positions[:, :, i, :]
+            fl = np.array(old.reshape(-1, 3))
+            v = np.full_like(fl, vinf)
+            v += np.vecdot(O.ref_induction_matrix(bound, pos, fl, C3["tol"]), gamma[None, :, None], axis=1)
+            ow.wake_positions[:, :, i + 1, :] = old + v.reshape(old.shape) * dt
+        ow.circulation[:, 1:] = ow.circulation[:, :-1]
+        ow.circulation[:, 0] = 0
+        m = ow.adjacent_surfaces != 0x7FFFFFFF
+        ow.circulation[m[:, 1], 0] += gamma[ow.adjacent_surfaces[m[:, 1], 1]]
+        ow.circulation[m[:, 0], 0] -= gamma[ow.adjacent_surfaces[m[:, 0], 0]]
+        ow.step_count += 1
+    elapsed = time.perf_counter() - t0
+    return {
+        "value": round(n_steps / elapsed, 4),
+        "unit": "steps/s",
+        "cores": os.cpu_count(),
+        "kind": "reference",
+        "sample": f"{n_steps} full-wake steps with the reference's convection (bound induction only), the "
+                  "reference's own src/core C (compute_line_induction + line_induction_to_*_surface_induction "
+                  "with its full line buffer, built unmodified into oracle/_ref), its Python step order restated "
+                  f"in numpy, scipy LAPACK LU; {elapsed:.2f} s",
+    }
+
+
+def run_c4(do_cpu: bool) -> dict:
+    """16k-panel closed-body solve: ellipsoid 4:1:0.5, 128 x 128 (16,384 panels), 16 RHS."""
+    from pyvl_amd import DeviceArray, _lib
+    from pyvl_amd import geometry as G
+    from pyvl_amd import linalg as LA
+    from pyvl_amd.cvl import Mesh
+
+    lib = _lib.load()
+    pos, el = G.ellipsoid(128, 128, axes=(4.0, 1.0, 0.5), sigma_rel=1e-3, seed=0)
+    mesh = Mesh(pos.shape[0], el)
+    n = mesh.n_surfaces
+    dp = DeviceArray.from_numpy(pos)
+    nrm = mesh.surface_normal(dp)
+    cps = mesh.surface_average_vec3(dp)
+    A = DeviceArray((n, n))
+    mesh.induction_matrix3(1e-6, dp, cps, nrm, A)  # warm
+    _lib.call("pvl_synchronize")
+    lib.pvl_prof_reset()
+    lib.pvl_prof_enable(1)
+    mesh.induction_matrix3(1e-6, dp, cps, nrm, A)
+    _lib.call("pvl_synchronize")
+    asm_ms, _, asm_pairs = prof_get(lib, "induction_matrix3")
+    lib.pvl_prof_enable(0)
+    diag = np.empty(n)
+    # deflation constant c = mean |diag A| (host copy of the diagonal)
+    Ah = A.numpy()
+    diag = np.diag(Ah).copy()
+    c = float(np.mean(np.abs(diag)))
+    LA.deflate(A, 0, n, c)
+    dirs = np.stack([G.cone_directions(1, 10.0, seed)[0] for seed in range(1, 17)])
+    nh = nrm.numpy()
+    B = np.ascontiguousarray(-(nh @ dirs.T))  # b_q = n . -(V_q)
+    dB = DeviceArray.from_numpy(B)
+
+    def mean_free(x):
+        return x - x.mean(axis=0, keepdims=True)
+
+    # plain FP64
+    lu64 = LA.LU(n, LA.FP64)
+    X64 = DeviceArray((n, 16))
+    lu64.factor(A)
+    _lib.call("pvl_synchronize")
+    t0 = time.perf_counter()
+    lu64.factor(A)
+    _lib.call("pvl_memcpy_d2d", X64.cptr, dB.cptr, C.c_size_t(dB.nbytes))
+    lu64.solve(X64)
+    _lib.call("pvl_synchronize")
+    t64 = time.perf_counter() - t0
+    x64 = mean_free(X64.numpy())
+    log(f"C4 FP64 LU+solve (16 RHS): {t64 * 1e3:.1f} ms")
+    # FP32 LU (MFMA) + FP64 refinement
+    lu32 = LA.LU(n, LA.FP32)
+    lu32.factor(A)
+    X32 = DeviceArray((n, 16))
+    _lib.call("pvl_synchronize")
+    t0 = time.perf_counter()
+    lu32.factor(A)
+    _, iters, _, ratios = LA.solve_refine(A, dB, lu32, tol=1e-13, out=X32)
+    _lib.call("pvl_synchronize")
+    t32 = time.perf_counter() - t0
+    x32 = mean_free(X32.numpy())
+    log(f"C4 FP32 LU + FP64 IR (16 RHS): {t32 * 1e3:.1f} ms, {iters} refinement steps")
+    rel = lambda a, b: float(np.max(np.linalg.norm(a - b, axis=0) / np.linalg.norm(b, axis=0)))  # noqa: E731
+    out = {
+        "n": n,
+        "assembly_fp64_ms": round(asm_ms, 3),
+        "assembly_alg_tflops": round(asm_pairs * FLOP_PER_PAIR / (asm_ms * 1e-3) / 1e12, 3),
+        "fp64_lu_solve_s": round(t64, 4),
+        "fp32_mfma_lu_ir_s": round(t32, 4),
+        "ir_steps": iters,
+        "ir_final_max_dx_over_x": float(np.max(ratios)),
+        "rel_err_fp32ir_vs_fp64gpu": rel(x32, x64),
+        "deflation_c": c,
+        "speedup_fp32ir_over_fp64": round(t64 / t32, 3),
+    }
+    if do_cpu:
+        import scipy.linalg as la
+
+        Ad = A.numpy()
+        t0 = time.perf_counter()
+        xr = la.lu_solve(la.lu_factor(Ad, overwrite_a=True, check_finite=False), B, check_finite=False)
+        tcpu = time.perf_counter() - t0
+        xr = mean_free(xr)
+        out.update({
+            "cpu_lapack_fp64_s": round(tcpu, 3),
+            "cpu_cores": os.cpu_count(),
+            "rel_err_fp64gpu_vs_lapack": rel(x64, xr),
+            "rel_err_fp32ir_vs_lapack": rel(x32, xr),
+        })
+        log(f"C4 scipy LAPACK FP64 on {os.cpu_count()} cores: {tcpu:.2f} s")
+    return out
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the host baselines")
+    ap.add_argument("--headline-only", action="store_true", help="only the FP64 free-wake workload")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus}); "
+              "the way the workload would shard is described in DESIGN.md", file=sys.stderr)
+        return 2
+    from pyvl_amd import _lib
+
+    _lib.call("pvl_init", 0)
+    head = run_c3("free", 0, args.steps, args.warmup, profile=True)
+    extra = {"c3_grow_400_steps_s": round(head["grow_400_s"], 3)}
+    prof = {k: {"ms": round(v[0], 3), "launches": v[1]} for k, v in head["prof"].items()}
+    extra["c3_timed_region_kernels"] = prof
+    cpu = None
+    if not args.headline_only:
+        f32 = run_c3("free", 1, args.steps, args.warmup, profile=False)
+        extra["c3_free_wake_fp32_induction_steps_per_s"] = round(f32["steps_per_s"], 3)
+        extra["c3_free_wake_fp32_induction_grow_400_s"] = round(f32["grow_400_s"], 3)
+        ref = run_c3("reference", 0, args.steps, args.warmup, profile=False)
+        extra["c3_reference_semantics_steps_per_s"] = round(ref["steps_per_s"], 3)
+        extra["c3_reference_semantics_grow_400_s"] = round(ref["grow_400_s"], 3)
+        extra["solve_16k"] = run_c4(not args.no_cpu)
+        if not args.no_cpu:
+            extra["cpu_reference_semantics"] = cpu_reference_step(ref, 2)
+            log(f"CPU reference semantics: {extra['cpu_reference_semantics']}")
+    if not args.no_cpu:
+        cpu = cpu_free_wake_step(head)
+        log(f"CPU free wake: {cpu}")
+    line = {
+        "metric": "wake steps/s (1k bound+25k wake panels)",
+        "value": round(head["steps_per_s"], 3),
+        "unit": "steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(head["ms_per_step"], 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": "unsteady free wake: flat wing 64x16 = 1,024 bound panels (span 8, chord 1), heave 0.1c, "
+                        "k=0.3, dt=c/16U, 400-row wake = 25,600 wake panels (26,065 nodes, 51,664 segments), "
+                        "tol 1e-6, FP64; timed steps run with the wake full",
+            "bound_panels": 1024,
+            "wake_panels": 25600,
+            "line_rows": 400,
+        },
+        "roofline": roofline_from(head["prof"]),
+        "cpu_baseline": cpu,
+        "extra": extra,
+    }
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -41,8 +41,15 @@ enum pvl_status
 
 enum pvl_precision
 {
-    PVL_FP64 = 0, /* reference arithmetic: FP64 throughout, reference operation order, no FMA */
-    PVL_FP32 = 1, /* mixed: FP64 geometry, cut-off tests and h^2; FP32 transcendental part */
+    /* FP64 throughout.  Dot products, h^2 and the cut-off test in the reference's operation
+     * order without FMA; the subtended angle in the single-angle form
+     * atan2((t2-t1) h, h^2 + t1 t2) with this library's atan2. */
+    PVL_FP64 = 0,
+    /* mixed: FP64 geometry, cut-off test, h^2 and cross product; FP32 angle (~1e-7). */
+    PVL_FP32 = 1,
+    /* FP64 with the reference's own angle expression atan2(t2,h) - atan2(t1,h) (two calls of
+     * the HIP math library's atan2): the literal restatement, kept for cross-checks. */
+    PVL_FP64_REF = 2,
 };
 
 /* ------------------------------------------------------------------------------------
@@ -64,6 +71,8 @@ int pvl_memset_zero(void *dev_dst, size_t bytes);
 int pvl_prof_enable(int on);
 int pvl_prof_reset(void);
 int pvl_prof_query(const char *name, double *total_ms, long *count);
+/* Work units accumulated under `name` (for induction kernels: (line, point) pairs). */
+int pvl_prof_work(const char *name, double *work);
 /* Generic event timer on the library stream (for whole-step wall times). */
 int pvl_timer_start(void);
 int pvl_timer_stop(double *ms);

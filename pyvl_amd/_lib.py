@@ -27,6 +27,7 @@ PVL_ERR_NODEV = -6
 
 PVL_FP64 = 0
 PVL_FP32 = 1
+PVL_FP64_REF = 2
 
 # name -> (restype, argtypes)
 SIGNATURES: dict[str, tuple] = {
@@ -43,6 +44,7 @@ SIGNATURES: dict[str, tuple] = {
     "pvl_prof_enable": (C.c_int, [C.c_int]),
     "pvl_prof_reset": (C.c_int, []),
     "pvl_prof_query": (C.c_int, [C.c_char_p, C.POINTER(C.c_double), C.POINTER(C.c_long)]),
+    "pvl_prof_work": (C.c_int, [C.c_char_p, C.POINTER(C.c_double)]),
     "pvl_timer_start": (C.c_int, []),
     "pvl_timer_stop": (C.c_int, [C.POINTER(C.c_double)]),
     "pvl_mesh_from_elements": (

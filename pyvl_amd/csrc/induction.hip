@@ -5,6 +5,20 @@
 #include "pvl_internal.hpp"
 #include "induction_core.hpp"
 
+#include <cstring>
+#include <type_traits>
+
+// Calls f(std::integral_constant<int, P>{}) for the precision form P selected at run time.
+template <class F> static void dispatch_precision(int precision, F &&f)
+{
+    if (precision == PVL_FP32)
+        f(std::integral_constant<int, PVL_FP32>{});
+    else if (precision == PVL_FP64_REF_FORM)
+        f(std::integral_constant<int, PVL_FP64_REF_FORM>{});
+    else
+        f(std::integral_constant<int, PVL_FP64>{});
+}
+
 using namespace pvl;
 
 namespace
@@ -220,7 +234,8 @@ __global__ void __launch_bounds__(256) line_circulation_kernel(unsigned n_lines,
 // One lane per point; the workgroup's four waves split a chunk of lines, partial sums
 // are combined in LDS in wave order and written per chunk; a second pass adds the chunks
 // in chunk order.  No atomics: the result does not depend on scheduling.
-template <int PREC>
+// SITE only gives the free-wake self-induction launch its own symbol for profilers.
+template <int PREC, int SITE>
 __global__ void __launch_bounds__(kBlock)
     point_velocity_kernel(unsigned n_pts, const double *__restrict__ pts, unsigned n_lines,
                           const LineGeom *__restrict__ geom, const double *__restrict__ gamma,
@@ -388,7 +403,7 @@ int check_common(const pvl_mesh *m, double tol, int precision)
         set_error("null mesh");
         return PVL_ERR_ARG;
     }
-    if (precision != PVL_FP64 && precision != PVL_FP32)
+    if (precision != PVL_FP64 && precision != PVL_FP32 && precision != PVL_FP64_REF_FORM)
     {
         set_error("unknown precision %d", precision);
         return PVL_ERR_ARG;
@@ -413,13 +428,11 @@ int launch_surface(const pvl_mesh *m, double tol, const double *pos, unsigned n_
         set_error("too many control points (%u) for one launch", n_cpts);
         return PVL_ERR_ARG;
     }
-    ProfScope prof(NORMAL ? "induction_matrix3" : "induction_matrix");
-    if (precision == PVL_FP64)
-        surface_induction_kernel<NORMAL, PVL_FP64, spw><<<grid, kBlock, 0, stream()>>>(
+    ProfScope prof(NORMAL ? "induction_matrix3" : "induction_matrix", (double)n_cpts * (double)m->n_lines);
+    dispatch_precision(precision, [&](auto P) {
+        surface_induction_kernel<NORMAL, decltype(P)::value, spw><<<grid, kBlock, 0, stream()>>>(
             n_cpts, m->n_surfaces, m->d_offsets, m->d_entries, g, cpts, normals, out, tol);
-    else
-        surface_induction_kernel<NORMAL, PVL_FP32, spw><<<grid, kBlock, 0, stream()>>>(
-            n_cpts, m->n_surfaces, m->d_offsets, m->d_entries, g, cpts, normals, out, tol);
+    });
     PVL_HIP(hipGetLastError());
     return PVL_OK;
 }
@@ -433,7 +446,8 @@ namespace pvl
 // entry point; the wake passes the prefix of lines its filled rows use -- the other lines
 // carry zero circulation).
 int point_velocity_impl(const pvl_mesh *m, double tol, const double *pos, const double *circ, unsigned n_pts,
-                        const double *pts, double *out, int accumulate, int precision, unsigned n_active)
+                        const double *pts, double *out, int accumulate, int precision, unsigned n_active,
+                        const char *prof_name)
 {
     if (n_pts == 0)
         return PVL_OK;
@@ -471,13 +485,16 @@ int point_velocity_impl(const pvl_mesh *m, double tol, const double *pos, const 
         return PVL_ERR_NOMEM;
     const dim3 grid(pblocks, n_chunks);
     {
-        ProfScope prof("point_velocity");
-        if (precision == PVL_FP64)
-            point_velocity_kernel<PVL_FP64>
-                <<<grid, kBlock, 0, stream()>>>(n_pts, pts, n_active, g, gamma, lpc, partial, tol);
-        else
-            point_velocity_kernel<PVL_FP32>
-                <<<grid, kBlock, 0, stream()>>>(n_pts, pts, n_active, g, gamma, lpc, partial, tol);
+        ProfScope prof(prof_name, (double)n_pts * (double)n_active);
+        const bool self_site = std::strcmp(prof_name, "wake_self_velocity") == 0;
+        dispatch_precision(precision, [&](auto P) {
+            if (self_site)
+                point_velocity_kernel<decltype(P)::value, 1>
+                    <<<grid, kBlock, 0, stream()>>>(n_pts, pts, n_active, g, gamma, lpc, partial, tol);
+            else
+                point_velocity_kernel<decltype(P)::value, 0>
+                    <<<grid, kBlock, 0, stream()>>>(n_pts, pts, n_active, g, gamma, lpc, partial, tol);
+        });
         PVL_HIP(hipGetLastError());
     }
     chunk_reduce_kernel<<<(3 * n_pts + 255) / 256, 256, 0, stream()>>>(n_pts, n_chunks, partial, out, accumulate);
@@ -522,11 +539,11 @@ int pvl_line_induction_matrix(const pvl_mesh *m, double tol, const double *pos, 
         set_error("too many control points (%u) for one launch", n_cpts);
         return PVL_ERR_ARG;
     }
-    ProfScope prof("line_induction_matrix");
-    if (precision == PVL_FP64)
-        line_induction_kernel<PVL_FP64, lpw><<<grid, kBlock, 0, stream()>>>(n_cpts, m->n_lines, g, cpts, out, tol);
-    else
-        line_induction_kernel<PVL_FP32, lpw><<<grid, kBlock, 0, stream()>>>(n_cpts, m->n_lines, g, cpts, out, tol);
+    ProfScope prof("line_induction_matrix", (double)n_cpts * (double)m->n_lines);
+    dispatch_precision(precision, [&](auto P) {
+        line_induction_kernel<decltype(P)::value, lpw>
+            <<<grid, kBlock, 0, stream()>>>(n_cpts, m->n_lines, g, cpts, out, tol);
+    });
     PVL_HIP(hipGetLastError());
     return PVL_OK;
 }

@@ -5,20 +5,116 @@
 //   K = cross(dr1, e) * (atan2(t2, h) - atan2(t1, h)) / h   otherwise
 //
 // |K| is the angle the segment subtends from p (no 1/(4 pi) factor).  The file must be
-// compiled with -ffp-contract=off: the FP64 form below then rounds exactly as the
-// reference's GCC x86-64 build (dot products left to right, no FMA), except for the
-// last-bit differences between this library's atan2 and glibc's.
+// compiled with -ffp-contract=off: the dot products, h^2 and the cut-off test below then
+// round exactly as the reference's GCC x86-64 build (left to right, no FMA), so the cut-off
+// branch is decided identically and the cancellation in h^2 near a segment's line (which
+// the reference's values carry) is reproduced.
+//
+// Angle forms:
+//   PVL_FP64_REF  the reference's expression: two FP64 atan2 calls of the HIP math library.
+//   PVL_FP64      one angle: atan2(t2, h) - atan2(t1, h) = atan2((t2 - t1) h, h^2 + t1 t2).
+//                 Both sides lie in (-pi, pi): with r_i = sqrt(h^2 + t_i^2) > 0, the sine and
+//                 cosine of the difference are (t2 - t1) h / (r1 r2) and (h^2 + t1 t2) /
+//                 (r1 r2).  The error of this form is O(eps) relative to the angle everywhere,
+//                 whereas the difference of two atan2 values loses digits in the far field
+//                 (|angle| << pi); the two forms agree to that loss (~1e-14 at r/len ~ 100).
+//                 atan2 is this library's own (pvl_atan2 below), 1/h comes from a
+//                 Newton-refined reciprocal square root.
+//   PVL_FP32      as PVL_FP64 with the angle, h and 1/h in FP32 (cut-off, dots, h^2 and
+//                 cross(dr1, e) stay FP64); ~1e-7 relative.
 #pragma once
 
 #include "pvl_internal.hpp"
 
-// PVL_FP64: reference operation order, two FP64 atan2 calls.
-// PVL_FP32: the cut-off test, the dot products, h^2 and cross(dr1, e) stay in FP64; the
-//   angle is one FP32 atan2 of the equivalent single-angle form
-//     atan2(t2, h) - atan2(t1, h) = atan2((t2 - t1) h, h^2 + t1 t2)
-//   (both sides lie in (-pi, pi): sin and cos of the difference are (t2 - t1) h / (r1 r2)
-//   and (h^2 + t1 t2) / (r1 r2) with r_i = sqrt(h^2 + t_i^2) > 0), which has no
-//   cancellation in the far field, so the FP32 angle keeps ~1e-7 relative accuracy.
+enum
+{
+    PVL_FP64_REF_FORM = 2
+};
+
+// atan(a) = a + a s P(s), s = a^2, 0 <= a <= 1.  Minimax coefficients (maximum relative
+// error of the polynomial 2.8e-17 for FP64, 1.7e-8 for FP32) fitted by
+// pyvl_amd/csrc/tools/fit_atan.py (Lawson iteration in 60-digit arithmetic).
+__device__ __forceinline__ double atan_poly64(double a)
+{
+    const double s = a * a;
+    double p = -2.0258217196429153806e-5;
+    p = fma(p, s, 2.2301921440372192015e-4);
+    p = fma(p, s, -1.1640578327914545487e-3);
+    p = fma(p, s, 3.855937704218664755e-3);
+    p = fma(p, s, -9.1844913587584364515e-3);
+    p = fma(p, s, 1.6977946435648326745e-2);
+    p = fma(p, s, -2.5826708868270838343e-2);
+    p = fma(p, s, 3.4067745269857893501e-2);
+    p = fma(p, s, -4.0926344607047341725e-2);
+    p = fma(p, s, 4.6739479480120956234e-2);
+    p = fma(p, s, -5.2392324387842287365e-2);
+    p = fma(p, s, 5.877307626399871717e-2);
+    p = fma(p, s, -6.6658603353628207695e-2);
+    p = fma(p, s, 7.6922129266775500417e-2);
+    p = fma(p, s, -9.0909012350179866105e-2);
+    p = fma(p, s, 1.1111110678724637807e-1);
+    p = fma(p, s, -1.4285714271333841475e-1);
+    p = fma(p, s, 1.999999999975499914e-1);
+    p = fma(p, s, -3.3333333333331858665e-1);
+    return fma(a * s, p, a);
+}
+
+__device__ __forceinline__ float atan_poly32(float a)
+{
+    const float s = a * a;
+    float p = 2.9206460455115316945e-3f;
+    p = fmaf(p, s, -1.6367743921648625112e-2f);
+    p = fmaf(p, s, 4.3211563153423941466e-2f);
+    p = fmaf(p, s, -7.5521892793692703723e-2f);
+    p = fmaf(p, s, 1.0665993049895164418e-1f);
+    p = fmaf(p, s, -1.4211052413584158043e-1f);
+    p = fmaf(p, s, 1.9993772488057382827e-1f);
+    p = fmaf(p, s, -3.3333152721644904729e-1f);
+    return fmaf(a * s, p, a);
+}
+
+// atan2(y, x) for finite arguments not both zero: a = min(|x|,|y|) / max(|x|,|y|) in [0, 1]
+// (Newton-refined reciprocal, quotient corrected by its residual), atan(a) from the
+// polynomial, then the octant: pi/2 - . when |y| > |x|, pi - . when x < 0, sign of y.
+__device__ __forceinline__ double pvl_atan2(double y, double x)
+{
+    const double ax = fabs(x), ay = fabs(y);
+    const double mn = fmin(ax, ay), mx = fmax(ax, ay);
+    double r = __builtin_amdgcn_rcp(mx);
+    double e = fma(-mx, r, 1.0);
+    r = fma(r, e, r);
+    e = fma(-mx, r, 1.0);
+    r = fma(r, e, r);
+    double a = mn * r;
+    a = fma(r, fma(-mx, a, mn), a);
+    double t = atan_poly64(a);
+    t = (ay > ax) ? (1.57079632679489661923 - t) : t;
+    t = (x < 0.0) ? (3.14159265358979323846 - t) : t;
+    return copysign(t, y);
+}
+
+__device__ __forceinline__ float pvl_atan2f(float y, float x)
+{
+    const float ax = fabsf(x), ay = fabsf(y);
+    const float mn = fminf(ax, ay), mx = fmaxf(ax, ay);
+    const float a = mn * __builtin_amdgcn_rcpf(mx);
+    float t = atan_poly32(a);
+    t = (ay > ax) ? (1.57079632679489661923f - t) : t;
+    t = (x < 0.0f) ? (3.14159265358979323846f - t) : t;
+    return copysignf(t, y);
+}
+
+// 1/sqrt(x) for x > 0 normal: hardware estimate and two Newton steps.
+__device__ __forceinline__ double pvl_rsqrt(double x)
+{
+    double y = __builtin_amdgcn_rsq(x);
+    double e = fma(-x * y, y, 1.0);
+    y = fma(0.5 * y, e, y);
+    e = fma(-x * y, y, 1.0);
+    y = fma(0.5 * y, e, y);
+    return y;
+}
+
 template <int PREC>
 __device__ __forceinline__ void line_k(const LineGeom &g, double px, double py, double pz, double tol, double tol2,
                                        double &kx, double &ky, double &kz)
@@ -33,17 +129,25 @@ __device__ __forceinline__ void line_k(const LineGeom &g, double px, double py, 
     const bool cut = (g.len < tol) || (h2 < tol2);
     const double hh = cut ? 1.0 : h2;
     double m;
-    if constexpr (PREC == PVL_FP64)
+    if constexpr (PREC == PVL_FP64_REF_FORM)
     {
         const double h = sqrt(hh);
         m = (atan2(t2, h) - atan2(t1, h)) / h;
     }
+    else if constexpr (PREC == PVL_FP64)
+    {
+        const double ih = pvl_rsqrt(hh);
+        const double h = hh * ih;
+        const double dphi = pvl_atan2((t2 - t1) * h, fma(t1, t2, hh));
+        m = dphi * ih;
+    }
     else
     {
         const float hf = sqrtf((float)hh);
+        const float ihf = __builtin_amdgcn_rcpf(hf);
         const float num = (float)(t2 - t1) * hf;
-        const float den = (float)(hh + t1 * t2);
-        m = (double)(atan2f(num, den) / hf);
+        const float den = (float)fma(t1, t2, hh);
+        m = (double)(pvl_atan2f(num, den) * ihf);
     }
     const double cx = d1y * g.ez - d1z * g.ey;
     const double cy = d1z * g.ex - d1x * g.ez;

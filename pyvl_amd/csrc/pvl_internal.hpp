@@ -25,8 +25,9 @@ int ensure_init();
 struct ProfScope
 {
     const char *name;
+    double work;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
-    explicit ProfScope(const char *n);
+    explicit ProfScope(const char *n, double work_units = 0.0);
     ~ProfScope();
 };
 
@@ -48,8 +49,11 @@ struct pvl_mesh;
 namespace pvl
 {
 // Point-velocity sum restricted to the first n_active lines of the mesh.
+// prof_name labels the main kernel's launch for the event profiler; its work unit is the
+// number of (line, point) pairs evaluated.
 int point_velocity_impl(const pvl_mesh *m, double tol, const double *pos, const double *circ, unsigned n_pts,
-                        const double *pts, double *out, int accumulate, int precision, unsigned n_active);
+                        const double *pts, double *out, int accumulate, int precision, unsigned n_active,
+                        const char *prof_name = "point_velocity");
 } // namespace pvl
 
 // Line geometry precomputed once per call: start point, end point, unit direction and

@@ -106,21 +106,61 @@ void *scratch(int slot, size_t bytes)
 }
 
 // ---------------------------------------------------------------------------------------
-// Profiling
+// Profiling: event pairs are recorded on the library stream without waiting; elapsed times
+// are resolved when queried, so enabling it does not serialise the host with the GPU.
 struct ProfEntry
 {
     double ms = 0;
     long count = 0;
+    double work = 0;
+};
+struct Pending
+{
+    std::string name;
+    hipEvent_t ev0, ev1;
+    double work;
 };
 static bool g_prof = false;
 static std::map<std::string, ProfEntry> g_prof_tab;
+static std::vector<Pending> g_pending;
+static std::vector<hipEvent_t> g_event_pool;
 
-ProfScope::ProfScope(const char *n) : name(n)
+static hipEvent_t take_event()
+{
+    if (!g_event_pool.empty())
+    {
+        hipEvent_t e = g_event_pool.back();
+        g_event_pool.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    hipEventCreate(&e);
+    return e;
+}
+
+static void resolve_pending()
+{
+    for (auto &p : g_pending)
+    {
+        hipEventSynchronize(p.ev1);
+        float ms = 0;
+        hipEventElapsedTime(&ms, p.ev0, p.ev1);
+        auto &e = g_prof_tab[p.name];
+        e.ms += ms;
+        e.count += 1;
+        e.work += p.work;
+        g_event_pool.push_back(p.ev0);
+        g_event_pool.push_back(p.ev1);
+    }
+    g_pending.clear();
+}
+
+ProfScope::ProfScope(const char *n, double work_units) : name(n), work(work_units)
 {
     if (!g_prof)
         return;
-    hipEventCreate(&ev0);
-    hipEventCreate(&ev1);
+    ev0 = take_event();
+    ev1 = take_event();
     hipEventRecord(ev0, g_stream);
 }
 
@@ -129,14 +169,9 @@ ProfScope::~ProfScope()
     if (!ev0)
         return;
     hipEventRecord(ev1, g_stream);
-    hipEventSynchronize(ev1);
-    float ms = 0;
-    hipEventElapsedTime(&ms, ev0, ev1);
-    auto &e = g_prof_tab[name];
-    e.ms += ms;
-    e.count += 1;
-    hipEventDestroy(ev0);
-    hipEventDestroy(ev1);
+    g_pending.push_back({name, ev0, ev1, work});
+    if (g_pending.size() > 4096)
+        resolve_pending();
 }
 
 } // namespace pvl
@@ -253,12 +288,14 @@ int pvl_prof_enable(int on)
 
 int pvl_prof_reset(void)
 {
+    resolve_pending();
     g_prof_tab.clear();
     return PVL_OK;
 }
 
 int pvl_prof_query(const char *name, double *total_ms, long *count)
 {
+    resolve_pending();
     auto it = g_prof_tab.find(name);
     if (it == g_prof_tab.end())
     {
@@ -268,6 +305,14 @@ int pvl_prof_query(const char *name, double *total_ms, long *count)
     }
     *total_ms = it->second.ms;
     *count = it->second.count;
+    return PVL_OK;
+}
+
+int pvl_prof_work(const char *name, double *work)
+{
+    resolve_pending();
+    auto it = g_prof_tab.find(name);
+    *work = it == g_prof_tab.end() ? 0.0 : it->second.work;
     return PVL_OK;
 }
 

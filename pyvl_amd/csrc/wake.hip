@@ -369,9 +369,8 @@ int pvl_wake_update(pvl_wake *w, double time, const pvl_mesh *bound, const doubl
     const unsigned n_old = cr * n_te;
     PVL_HIP(hipMemcpyAsync(w->d_old, w->d_upos, sizeof(double) * 3 * n_old, hipMemcpyDeviceToDevice, stream()));
     {
-        ProfScope prof("wake_bound_velocity");
-        if (int st = pvl_point_velocity(bound, w->tol, bound_pos_dev, circulation_dev, n_old, w->d_old, w->d_vel, 0,
-                                        precision))
+        if (int st = point_velocity_impl(bound, w->tol, bound_pos_dev, circulation_dev, n_old, w->d_old, w->d_vel, 0,
+                                         precision, bound->n_lines, "wake_bound_velocity"))
             return st;
     }
     if (mode == 1)
@@ -379,9 +378,8 @@ int pvl_wake_update(pvl_wake *w, double time, const pvl_mesh *bound, const doubl
         const unsigned na = active_lines(w);
         if (na)
         {
-            ProfScope prof("wake_self_velocity");
             if (int st = point_velocity_impl(w->full, w->tol, w->d_upos, w->d_circ[w->cur], n_old, w->d_old, w->d_vel,
-                                             1, precision, na))
+                                             1, precision, na, "wake_self_velocity"))
                 return st;
         }
     }
@@ -426,7 +424,6 @@ int pvl_wake_apply_corrections(pvl_wake *w, unsigned n_cpts, const double *cpts_
     if (!W0)
         return PVL_ERR_NOMEM;
     {
-        ProfScope prof("wake_implicit");
         if (int st = pvl_induction_matrix3(w->row0, w->tol, w->d_upos, n_cpts, cpts_dev, normals_dev, W0, precision))
             return st;
         implicit_kernel<<<(n_cpts + 255) / 256, 256, 0, stream()>>>(mat_dev, n_cpts, n_surf, W0, w->n_shed,
@@ -444,9 +441,8 @@ int pvl_wake_apply_corrections(pvl_wake *w, unsigned n_cpts, const double *cpts_
     if (!v)
         return PVL_ERR_NOMEM;
     {
-        ProfScope prof("wake_explicit");
         if (int st = point_velocity_impl(w->full, w->tol, w->d_upos, w->d_circ_x, n_cpts, cpts_dev, v, 0, precision,
-                                         na))
+                                         na, "wake_explicit"))
             return st;
         explicit_kernel<<<(n_cpts + 255) / 256, 256, 0, stream()>>>(rhs_dev, normals_dev, v, n_cpts);
         PVL_HIP(hipGetLastError());

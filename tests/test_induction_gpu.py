@@ -12,7 +12,7 @@ import pytest
 from oracle import oracle as O
 from pyvl_amd import DeviceArray
 from pyvl_amd import geometry as G
-from pyvl_amd._lib import PVL_FP32, PVL_FP64
+from pyvl_amd._lib import PVL_FP32, PVL_FP64, PVL_FP64_REF
 from pyvl_amd.cvl import Mesh
 
 pytestmark = pytest.mark.gpu
@@ -66,8 +66,12 @@ def test_c1_flat_wing_assembly():
     assert np.max(np.abs(off)) < 1e-13
 
 
+FP64_FORMS = pytest.mark.parametrize("prec", [PVL_FP64, PVL_FP64_REF], ids=["fp64", "fp64_ref_form"])
+
+
+@FP64_FORMS
 @pytest.mark.parametrize("name,pos,el", CASES, ids=[c[0] for c in CASES])
-def test_assembly_parity(name, pos, el):
+def test_assembly_parity(name, pos, el, prec):
     m, t, n, c = _setup(pos, el)
     rng = np.random.default_rng(5)
     extra = rng.normal(0, 1.5, (77, 3))  # ragged count, points off the surface
@@ -75,30 +79,32 @@ def test_assembly_parity(name, pos, el):
     nn = rng.normal(size=cp.shape)
     nn /= np.linalg.norm(nn, axis=1)[:, None]
     want = O.induction_matrix3(t, pos, cp, nn, TOL)
-    got = m.induction_matrix3(TOL, pos, cp, nn)
+    got = m.induction_matrix3(TOL, pos, cp, nn, precision=prec)
     assert rel_max(got, want) <= 1e-12, name
     assert rel_fro(got, want) <= 1e-12, name
     # resident path gives the same bits as the staged one
-    d = m.induction_matrix3(TOL, DeviceArray.from_numpy(pos), DeviceArray.from_numpy(cp), DeviceArray.from_numpy(nn))
+    d = m.induction_matrix3(TOL, DeviceArray.from_numpy(pos), DeviceArray.from_numpy(cp), DeviceArray.from_numpy(nn),
+                            precision=prec)
     assert np.array_equal(d.numpy(), got)
     # caller-provided out and a valid line buffer are accepted (buffer unused)
     out = np.empty_like(got)
     buf = np.empty(m.n_lines * cp.shape[0] * 3)
-    assert m.induction_matrix3(TOL, pos, cp, nn, out, buf) is out
+    assert m.induction_matrix3(TOL, pos, cp, nn, out, buf, precision=prec) is out
     assert np.array_equal(out, got)
 
 
+@FP64_FORMS
 @pytest.mark.parametrize("name,pos,el", CASES, ids=[c[0] for c in CASES])
-def test_surface_and_line_tensors(name, pos, el):
+def test_surface_and_line_tensors(name, pos, el, prec):
     m, t, n, c = _setup(pos, el)
     rng = np.random.default_rng(6)
     cp = np.ascontiguousarray(np.concatenate([c[:50], pos[:9], rng.normal(0, 2, (13, 3))]))
     want = O.induction_matrix(t, pos, cp, TOL)
-    got = m.induction_matrix(TOL, pos, cp)
+    got = m.induction_matrix(TOL, pos, cp, precision=prec)
     assert got.shape == (cp.shape[0], m.n_surfaces, 3)
     assert rel_max(got, want) <= 1e-12
     wl = O.line_induction(t, pos, cp, TOL)
-    gl = m.line_induction_matrix(TOL, pos, cp)
+    gl = m.line_induction_matrix(TOL, pos, cp, precision=prec)
     assert gl.shape == (cp.shape[0], m.n_lines, 3)
     assert rel_max(gl, wl) <= 1e-12
     # cut-off branch decided identically: zeros exactly where the oracle has them
@@ -138,14 +144,15 @@ def test_reference_normal_and_center_expectations():
         assert np.mean(positions[e, :], axis=0) == pytest.approx(cv)
 
 
+@FP64_FORMS
 @pytest.mark.parametrize("name,pos,el", CASES, ids=[c[0] for c in CASES])
-def test_point_velocity_parity(name, pos, el):
+def test_point_velocity_parity(name, pos, el, prec):
     m, t, n, c = _setup(pos, el)
     rng = np.random.default_rng(8)
     circ = rng.normal(size=m.n_surfaces)
     pts = np.ascontiguousarray(np.concatenate([c + 0.05 * n, pos, rng.normal(0, 2, (100, 3))]))
     want = O.point_velocity(t, pos, circ, pts, TOL)
-    got = m.point_velocity(TOL, pos, circ, pts)
+    got = m.point_velocity(TOL, pos, circ, pts, precision=prec)
     # line-circulation form vs surface form: same sum, different association of the
     # rounding, so the difference is a few ulp of the largest velocity
     assert rel_max(got, want) <= 1e-12
