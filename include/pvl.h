@@ -209,6 +209,36 @@ int pvl_wake_set_state(pvl_wake *w, const double *positions_host, const double *
                        double current_time);
 
 /* ------------------------------------------------------------------------------------
+ * Learned warm start.  No reference counterpart (the reference solves every step with
+ * LAPACK).  A small MLP (BF16 operands on MFMA, FP32 accumulation, FP32 master weights and
+ * Adam state) predicts per-panel circulations; FP32 GMRES inside FP64 refinement starts from
+ * that prediction; LU is the caller's fallback.
+ * ---------------------------------------------------------------------------------- */
+typedef struct pvl_mlp pvl_mlp;
+/* n_layers linear layers: d_in -> width -> ... -> width -> 1, SiLU between layers. */
+int pvl_mlp_create(pvl_mlp **out, int d_in, int width, int n_layers);
+int pvl_mlp_destroy(pvl_mlp *h);
+int pvl_mlp_num_params(const pvl_mlp *h, long *n);
+/* Flat FP32 parameters, per layer W (out x in, row-major) then b (out). */
+int pvl_mlp_set_params(pvl_mlp *h, const float *params_host);
+int pvl_mlp_get_params(const pvl_mlp *h, float *params_host);
+/* Y[b] = net(X[b, :]); X (batch x d_in) and Y (batch) FP32 device buffers. */
+int pvl_mlp_forward(pvl_mlp *h, const float *X_dev, float *Y_dev, int batch);
+/* Gradient of the mean squared error against T (no update). */
+int pvl_mlp_gradient(pvl_mlp *h, const float *X_dev, const float *T_dev, int batch, float *grads_host,
+                     float *loss_host);
+/* One Adam step with decoupled weight decay. */
+int pvl_mlp_train_step(pvl_mlp *h, const float *X_dev, const float *T_dev, int batch, float lr, float beta1,
+                       float beta2, float eps, float wd, float *loss_host);
+/* FP32 GMRES(m) (right Jacobi preconditioning) inside FP64 iterative refinement, starting
+ * from x_dev.  stats[0] = corrections, stats[1] = GMRES steps, stats[2] = converged flag.
+ * PVL_ERR_NOTCONV when max|dx| <= tol * max|x| was not reached within max_outer. */
+int pvl_gmres_refine(int n, const double *A64_dev, const float *A32_dev, const double *b_dev, double *x_dev, double tol,
+                     int m, double inner_tol, int max_outer, int *stats, double *final_ratio);
+/* FP32 copy of an FP64 device array. */
+int pvl_convert_f64_f32(const double *src_dev, float *dst_dev, size_t count);
+
+/* ------------------------------------------------------------------------------------
  * Step helpers of run_solver (python/pyvl/solver.py:199-206, :229-233).
  * ---------------------------------------------------------------------------------- */
 /* out_i = n_i . -(vinf - cpvel_i); cpvel_dev may be NULL (fixed body). */

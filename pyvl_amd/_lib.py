@@ -118,6 +118,24 @@ SIGNATURES: dict[str, tuple] = {
     "pvl_wake_get_velocity": (C.c_int, [C.c_void_p, C.c_uint, _dp, _dp, C.c_int, C.c_int]),
     "pvl_wake_get_state": (C.c_int, [C.c_void_p, _hp, _hp]),
     "pvl_wake_set_state": (C.c_int, [C.c_void_p, _hp, _hp, C.c_uint, C.c_double]),
+    "pvl_mlp_create": (C.c_int, [C.POINTER(C.c_void_p), C.c_int, C.c_int, C.c_int]),
+    "pvl_mlp_destroy": (C.c_int, [C.c_void_p]),
+    "pvl_mlp_num_params": (C.c_int, [C.c_void_p, C.POINTER(C.c_long)]),
+    "pvl_mlp_set_params": (C.c_int, [C.c_void_p, _hp]),
+    "pvl_mlp_get_params": (C.c_int, [C.c_void_p, _hp]),
+    "pvl_mlp_forward": (C.c_int, [C.c_void_p, _dp, _dp, C.c_int]),
+    "pvl_mlp_gradient": (C.c_int, [C.c_void_p, _dp, _dp, C.c_int, _hp, C.POINTER(C.c_float)]),
+    "pvl_mlp_train_step": (
+        C.c_int,
+        [C.c_void_p, _dp, _dp, C.c_int, C.c_float, C.c_float, C.c_float, C.c_float, C.c_float,
+         C.POINTER(C.c_float)],
+    ),
+    "pvl_gmres_refine": (
+        C.c_int,
+        [C.c_int, _dp, _dp, _dp, _dp, C.c_double, C.c_int, C.c_double, C.c_int, C.POINTER(C.c_int),
+         C.POINTER(C.c_double)],
+    ),
+    "pvl_convert_f64_f32": (C.c_int, [_dp, _dp, C.c_size_t]),
     "pvl_rhs_normal_velocity": (C.c_int, [C.c_uint, _dp, _dp, C.POINTER(C.c_double), _dp]),
     "pvl_remove_mean": (C.c_int, [_dp, C.c_int, C.c_int]),
 }

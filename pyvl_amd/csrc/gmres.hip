@@ -1,0 +1,473 @@
+// FP32 restarted GMRES inside FP64 iterative refinement, for the circulation system
+// A x = b of one step.  No reference counterpart: the reference factors A with LAPACK every
+// step (python/pyvl/solver.py:223-227); here a prediction of x (e.g. the learned one)
+// warm-starts the refinement, and a solve that does not converge reports so the caller can
+// fall back to LU.
+//
+// One GMRES(m) cycle is a single cooperative kernel: workgroup g owns a block of rows;
+// per Arnoldi step it forms w = A M^-1 v_j for its rows (M = diag(A), right Jacobi
+// preconditioning), orthogonalises with classical Gram-Schmidt applied twice (partial dot
+// products of all groups combined after a grid barrier), normalises, and every group
+// applies the same Givens rotations to its private copy of the Hessenberg column, so the
+// residual estimate and the stopping decision are identical everywhere without a host
+// round trip.  Four grid barriers per step (two Gram-Schmidt passes, the norm, and the new
+// basis vector).
+#include "pvl_internal.hpp"
+
+#include <cmath>
+#include <vector>
+
+using namespace pvl;
+
+namespace
+{
+
+constexpr int kMaxM = 48;          // restart length cap
+constexpr int kGroups = 128;       // cooperative grid size cap
+constexpr unsigned kSpin = 20000000u;
+
+__device__ __forceinline__ void gbar(unsigned *counter, unsigned target, int *err)
+{
+    __threadfence();
+    __syncthreads();
+    if (threadIdx.x == 0)
+    {
+        __hip_atomic_fetch_add(counter, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        unsigned spins = 0;
+        while (__hip_atomic_load(counter, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < target)
+        {
+            __builtin_amdgcn_s_sleep(1);
+            if (++spins > kSpin || __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+            {
+                __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                break;
+            }
+        }
+    }
+    __syncthreads();
+}
+
+__device__ __forceinline__ float ld(const float *p)
+{
+    return __builtin_bit_cast(float, __hip_atomic_load((const unsigned *)p, __ATOMIC_RELAXED,
+                                                       __HIP_MEMORY_SCOPE_AGENT));
+}
+__device__ __forceinline__ void st(float *p, float v)
+{
+    __hip_atomic_store((unsigned *)p, __builtin_bit_cast(unsigned, v), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+struct GmresX
+{
+    unsigned counter;
+    int err;
+    int pad[2];
+    float part[3][kGroups][kMaxM + 2]; // per-group partial sums, one buffer per barrier phase
+};
+
+// rows of group g: [g*rpg, min(n, (g+1)*rpg)); thread t handles rows r0 + t, r0 + t + 256, ...
+__global__ void __launch_bounds__(256) gmres_cycle_kernel(int n, const float *__restrict__ A,
+                                                          const float *__restrict__ dinv,
+                                                          const float *__restrict__ rhs, float *__restrict__ d,
+                                                          float *__restrict__ V, int m, float tol, int rpg,
+                                                          GmresX *__restrict__ X, int *__restrict__ result)
+{
+    __shared__ float Hs[kMaxM + 1][kMaxM];    // Hessenberg (this group's copy)
+    __shared__ float cs[kMaxM], sn[kMaxM], gv[kMaxM + 1];
+    __shared__ float red[kMaxM + 8];
+    __shared__ float hcol[kMaxM + 2];
+    __shared__ int s_steps;
+    __shared__ float s_beta;
+    const int tid = threadIdx.x, g = blockIdx.x, G = gridDim.x;
+    const int r0 = g * rpg, r1 = min(n, r0 + rpg);
+    unsigned bar = 0;
+    // local rows' w in registers: at most rpg / 256 rows per thread (rpg <= 256)
+    // 1. beta = ||rhs||, v0 = rhs / beta
+    float s = 0.0f;
+    for (int r = r0 + tid; r < r1; r += 256)
+        s += rhs[r] * rhs[r];
+    for (int off = 32; off > 0; off >>= 1)
+        s += __shfl_xor(s, off);
+    if ((tid & 63) == 0)
+        red[tid >> 6] = s;
+    __syncthreads();
+    if (tid == 0)
+        st(&X->part[0][g][0], red[0] + red[1] + red[2] + red[3]);
+    bar += G;
+    gbar(&X->counter, bar, &X->err);
+    if (tid == 0)
+    {
+        float tot = 0.0f;
+        for (int q = 0; q < G; ++q)
+            tot += ld(&X->part[0][q][0]);
+        s_beta = sqrtf(tot);
+        gv[0] = s_beta;
+    }
+    __syncthreads();
+    const float beta = s_beta;
+    if (!(beta > 0.0f))
+    {
+        for (int r = r0 + tid; r < r1; r += 256)
+            d[r] = 0.0f;
+        if (g == 0 && tid == 0)
+            result[0] = 0;
+        return;
+    }
+    for (int r = r0 + tid; r < r1; r += 256)
+        st(&V[r], rhs[r] / beta);
+    bar += G;
+    gbar(&X->counter, bar, &X->err);
+    int j = 0;
+    float resid = beta;
+    for (; j < m; ++j)
+    {
+        const float *vj = V + (size_t)j * n;
+        float *vn = V + (size_t)(j + 1) * n;
+        // w = A (dinv .* v_j) for own rows; one wave per row, lanes over columns
+        const int lane = tid & 63, wv = tid >> 6;
+        for (int r = r0 + wv; r < r1; r += 4)
+        {
+            const float *a = A + (size_t)r * n;
+            float acc = 0.0f;
+            for (int c = lane; c < n; c += 64)
+                acc = fmaf(a[c], dinv[c] * ld(&vj[c]), acc);
+            for (int off = 32; off > 0; off >>= 1)
+                acc += __shfl_xor(acc, off);
+            if (lane == 0)
+                vn[r] = acc; // staged in V[j+1] (own rows only)
+        }
+        __syncthreads();
+        // two passes of classical Gram-Schmidt: h_i = <w, v_i> summed over groups, w -= h_i v_i
+        for (int pass = 0; pass < 2; ++pass)
+        {
+            for (int i = wv; i <= j; i += 4)
+            {
+                float acc = 0.0f;
+                const float *vi = V + (size_t)i * n;
+                for (int r = r0 + lane; r < r1; r += 64)
+                    acc = fmaf(vn[r], ld(&vi[r]), acc);
+                for (int off = 32; off > 0; off >>= 1)
+                    acc += __shfl_xor(acc, off);
+                if (lane == 0)
+                    st(&X->part[pass][g][i], acc);
+            }
+            bar += G;
+            gbar(&X->counter, bar, &X->err);
+            for (int i = tid; i <= j; i += 256)
+            {
+                float tot = 0.0f;
+                for (int q = 0; q < G; ++q)
+                    tot += ld(&X->part[pass][q][i]);
+                red[i] = tot;
+                hcol[i] = pass == 0 ? tot : hcol[i] + tot;
+            }
+            __syncthreads();
+            for (int r = r0 + tid; r < r1; r += 256)
+            {
+                float wr = vn[r];
+                for (int i = 0; i <= j; ++i)
+                    wr = fmaf(-red[i], ld(&V[(size_t)i * n + r]), wr);
+                vn[r] = wr;
+            }
+            __syncthreads();
+        }
+        // ||w||
+        {
+            float acc = 0.0f;
+            for (int r = r0 + tid; r < r1; r += 256)
+                acc = fmaf(vn[r], vn[r], acc);
+            for (int off = 32; off > 0; off >>= 1)
+                acc += __shfl_xor(acc, off);
+            if ((tid & 63) == 0)
+                red[kMaxM + 1 + (tid >> 6)] = acc;
+            __syncthreads();
+            if (tid == 0)
+                st(&X->part[2][g][0], red[kMaxM + 1] + red[kMaxM + 2] + red[kMaxM + 3] + red[kMaxM + 4]);
+            bar += G;
+            gbar(&X->counter, bar, &X->err);
+            if (tid == 0)
+            {
+                float tot = 0.0f;
+                for (int q = 0; q < G; ++q)
+                    tot += ld(&X->part[2][q][0]);
+                hcol[j + 1] = sqrtf(tot);
+            }
+            __syncthreads();
+        }
+        const float hn = hcol[j + 1];
+        for (int r = r0 + tid; r < r1; r += 256)
+            st(&vn[r], hn > 0.0f ? vn[r] / hn : 0.0f);
+        // Givens on this group's copy
+        if (tid == 0)
+        {
+            for (int i = 0; i <= j + 1; ++i)
+                Hs[i][j] = hcol[i];
+            for (int i = 0; i < j; ++i)
+            {
+                const float a = Hs[i][j], b = Hs[i + 1][j];
+                Hs[i][j] = cs[i] * a + sn[i] * b;
+                Hs[i + 1][j] = -sn[i] * a + cs[i] * b;
+            }
+            const float a = Hs[j][j], b = Hs[j + 1][j];
+            const float rr = sqrtf(a * a + b * b);
+            cs[j] = rr > 0.0f ? a / rr : 1.0f;
+            sn[j] = rr > 0.0f ? b / rr : 0.0f;
+            Hs[j][j] = rr;
+            Hs[j + 1][j] = 0.0f;
+            gv[j + 1] = -sn[j] * gv[j];
+            gv[j] = cs[j] * gv[j];
+            s_steps = fabsf(gv[j + 1]) <= tol * beta || hn == 0.0f ? 1 : 0;
+        }
+        bar += G;
+        gbar(&X->counter, bar, &X->err);
+        resid = fabsf(gv[j + 1]);
+        if (s_steps)
+        {
+            ++j;
+            break;
+        }
+    }
+    if (j > m)
+        j = m;
+    // y = H^-1 g (upper triangular j x j), then d = dinv .* (V y) for own rows
+    __shared__ float y[kMaxM];
+    if (tid == 0)
+    {
+        for (int i = j - 1; i >= 0; --i)
+        {
+            float t = gv[i];
+            for (int k = i + 1; k < j; ++k)
+                t -= Hs[i][k] * y[k];
+            y[i] = t / Hs[i][i];
+        }
+    }
+    __syncthreads();
+    for (int r = r0 + tid; r < r1; r += 256)
+    {
+        float acc = 0.0f;
+        for (int i = 0; i < j; ++i)
+            acc = fmaf(y[i], ld(&V[(size_t)i * n + r]), acc);
+        d[r] = dinv[r] * acc;
+    }
+    if (g == 0 && tid == 0)
+    {
+        result[0] = j;
+        result[1] = __float_as_int(resid / beta);
+    }
+}
+
+__global__ void diag_inv_kernel(const float *__restrict__ A, int n, float *__restrict__ dinv)
+{
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i < n)
+    {
+        const float a = A[(size_t)i * n + i];
+        dinv[i] = a != 0.0f ? 1.0f / a : 1.0f;
+    }
+}
+
+__global__ void r64_to_32_kernel(const double *__restrict__ r, float *__restrict__ r32, int n, double scale)
+{
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i < n)
+        r32[i] = (float)(r[i] * scale);
+}
+
+__global__ void add_scaled_kernel(double *__restrict__ x, const float *__restrict__ d, int n, double scale,
+                                  double *__restrict__ norms)
+{
+    __shared__ double pd[256], px[256];
+    double md = 0.0, mx = 0.0;
+    for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256)
+    {
+        const double di = (double)d[i] * scale;
+        const double xi = x[i] + di;
+        x[i] = xi;
+        md = fmax(md, fabs(di));
+        mx = fmax(mx, fabs(xi));
+    }
+    pd[threadIdx.x] = md;
+    px[threadIdx.x] = mx;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1)
+    {
+        if ((int)threadIdx.x < w)
+        {
+            pd[threadIdx.x] = fmax(pd[threadIdx.x], pd[threadIdx.x + w]);
+            px[threadIdx.x] = fmax(px[threadIdx.x], px[threadIdx.x + w]);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+    {
+        // max-combine across blocks (non-negative doubles order like their bit patterns)
+        atomicMax((unsigned long long *)&norms[0], (unsigned long long)__double_as_longlong(pd[0]));
+        atomicMax((unsigned long long *)&norms[1], (unsigned long long)__double_as_longlong(px[0]));
+    }
+}
+
+__global__ void maxabs_kernel(const double *__restrict__ r, int n, double *__restrict__ out)
+{
+    __shared__ double p[256];
+    double m = 0.0;
+    for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256)
+        m = fmax(m, fabs(r[i]));
+    p[threadIdx.x] = m;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1)
+    {
+        if ((int)threadIdx.x < w)
+            p[threadIdx.x] = fmax(p[threadIdx.x], p[threadIdx.x + w]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        atomicMax((unsigned long long *)out, (unsigned long long)__double_as_longlong(p[0]));
+}
+
+struct GmresWs
+{
+    GmresX *x = nullptr;
+    int *result = nullptr;
+    int n_cap = 0, m_cap = 0;
+    float *V = nullptr, *r32 = nullptr, *d32 = nullptr, *dinv = nullptr;
+    double *r64 = nullptr, *norms = nullptr;
+};
+GmresWs g_gw;
+
+int ensure_gmres_ws(int n, int m)
+{
+    if (!g_gw.x)
+    {
+        PVL_HIP(hipMalloc(&g_gw.x, sizeof(GmresX)));
+        PVL_HIP(hipMalloc(&g_gw.result, 4 * sizeof(int)));
+        PVL_HIP(hipMalloc(&g_gw.norms, 4 * sizeof(double)));
+    }
+    if (n > g_gw.n_cap || m > g_gw.m_cap)
+    {
+        hipStreamSynchronize(stream());
+        hipFree(g_gw.V);
+        hipFree(g_gw.r32);
+        hipFree(g_gw.d32);
+        hipFree(g_gw.dinv);
+        hipFree(g_gw.r64);
+        PVL_HIP(hipMalloc(&g_gw.V, sizeof(float) * (size_t)(m + 1) * n));
+        PVL_HIP(hipMalloc(&g_gw.r32, sizeof(float) * n));
+        PVL_HIP(hipMalloc(&g_gw.d32, sizeof(float) * n));
+        PVL_HIP(hipMalloc(&g_gw.dinv, sizeof(float) * n));
+        PVL_HIP(hipMalloc(&g_gw.r64, sizeof(double) * n));
+        g_gw.n_cap = n;
+        g_gw.m_cap = m;
+    }
+    return PVL_OK;
+}
+
+} // namespace
+
+extern "C" {
+
+// x (FP64, n) holds the initial guess and receives the solution of A x = b.  A64 (FP64)
+// forms the residuals, A32 (FP32 copy of A) the inner GMRES(m) cycles, each run to relative
+// residual inner_tol or m steps.  Stops when max|dx| <= tol * max|x| after a correction; at
+// most max_outer corrections (each one restarts GMRES from the new FP64 residual).  stats[0] = corrections,
+// stats[1] = GMRES steps in total, stats[2] = 1 if converged.  Returns PVL_ERR_NOTCONV when
+// not converged (x then holds the last iterate).
+int pvl_gmres_refine(int n, const double *A64_dev, const float *A32_dev, const double *b_dev, double *x_dev, double tol,
+                     int m, double inner_tol, int max_outer, int *stats, double *final_ratio)
+{
+    if (int st = ensure_init())
+        return st;
+    if (n <= 0 || m <= 0 || m > kMaxM)
+    {
+        set_error("gmres_refine: n=%d, m=%d (1..%d)", n, m, kMaxM);
+        return PVL_ERR_ARG;
+    }
+    if (int st = ensure_gmres_ws(n, m))
+        return st;
+    const int G = std::min(kGroups, (n + 15) / 16);
+    const int rpg = (n + G - 1) / G;
+    if (rpg > 256 * 64)
+    {
+        set_error("gmres_refine: n=%d too large", n);
+        return PVL_ERR_ARG;
+    }
+    diag_inv_kernel<<<(n + 255) / 256, 256, 0, stream()>>>(A32_dev, n, g_gw.dinv);
+    PVL_HIP(hipGetLastError());
+    int total_steps = 0, outer = 0;
+    bool converged = false;
+    double ratio = INFINITY;
+    for (; outer < max_outer; ++outer)
+    {
+        if (int st = pvl_residual(n, A64_dev, x_dev, b_dev, g_gw.r64, 1))
+            return st;
+        // scale the residual to O(1) for the FP32 solve
+        PVL_HIP(hipMemsetAsync(g_gw.norms, 0, 4 * sizeof(double), stream()));
+        maxabs_kernel<<<64, 256, 0, stream()>>>(g_gw.r64, n, g_gw.norms + 2);
+        double rmax = 0;
+        PVL_HIP(hipMemcpyAsync(&rmax, g_gw.norms + 2, sizeof(double), hipMemcpyDeviceToHost, stream()));
+        PVL_HIP(hipStreamSynchronize(stream()));
+        if (rmax == 0.0)
+        {
+            converged = true;
+            ratio = 0.0;
+            break;
+        }
+        const double scale = 1.0 / rmax;
+        r64_to_32_kernel<<<(n + 255) / 256, 256, 0, stream()>>>(g_gw.r64, g_gw.r32, n, scale);
+        PVL_HIP(hipGetLastError());
+        // one GMRES(m) cycle on A d = r; each further correction restarts from the new residual
+        {
+            PVL_HIP(hipMemsetAsync(g_gw.x, 0, 16, stream()));
+            int nn = n, mm = m, rp = rpg;
+            float tl = (float)inner_tol;
+            const float *Ap = A32_dev, *dv = g_gw.dinv, *rh = g_gw.r32;
+            float *dd = g_gw.d32, *Vp = g_gw.V;
+            GmresX *Xp = g_gw.x;
+            int *res = g_gw.result;
+            void *args[] = {&nn, (void *)&Ap, (void *)&dv, (void *)&rh, &dd, &Vp, &mm, &tl, &rp, &Xp, &res};
+            {
+                ProfScope prof("gmres_cycle");
+                PVL_HIP(hipLaunchCooperativeKernel((const void *)gmres_cycle_kernel, dim3(G), dim3(256), args, 0,
+                                                   stream()));
+            }
+            PVL_HIP(hipMemsetAsync(g_gw.norms, 0, 2 * sizeof(double), stream()));
+            add_scaled_kernel<<<64, 256, 0, stream()>>>(x_dev, g_gw.d32, n, 1.0 / scale, g_gw.norms);
+            PVL_HIP(hipGetLastError());
+            int hres[2];
+            double hn[2];
+            int herr = 0;
+            PVL_HIP(hipMemcpyAsync(hres, g_gw.result, 2 * sizeof(int), hipMemcpyDeviceToHost, stream()));
+            PVL_HIP(hipMemcpyAsync(hn, g_gw.norms, 2 * sizeof(double), hipMemcpyDeviceToHost, stream()));
+            PVL_HIP(hipMemcpyAsync(&herr, &g_gw.x->err, sizeof(int), hipMemcpyDeviceToHost, stream()));
+            PVL_HIP(hipStreamSynchronize(stream()));
+            if (herr)
+            {
+                set_error("GMRES grid barrier timed out");
+                return PVL_ERR_HIP;
+            }
+            total_steps += hres[0];
+            ratio = hn[1] > 0 ? hn[0] / hn[1] : hn[0];
+        }
+        if (ratio <= tol)
+        {
+            converged = true;
+            ++outer;
+            break;
+        }
+    }
+    if (stats)
+    {
+        stats[0] = outer;
+        stats[1] = total_steps;
+        stats[2] = converged ? 1 : 0;
+    }
+    if (final_ratio)
+        *final_ratio = ratio;
+    if (!converged)
+    {
+        set_error("GMRES refinement did not reach %.1e (last max|dx|/max|x| = %.3e)", tol, ratio);
+        return PVL_ERR_NOTCONV;
+    }
+    return PVL_OK;
+}
+
+} // extern "C"

@@ -1043,6 +1043,17 @@ int pvl_solve_refine(pvl_lu *h, const double *A_dev, const double *B_dev, double
     return PVL_OK;
 }
 
+int pvl_convert_f64_f32(const double *src, float *dst, size_t count)
+{
+    if (int st = ensure_init())
+        return st;
+    if (count == 0)
+        return PVL_OK;
+    f64_to_f32_kernel<<<1024, 256, 0, stream()>>>(src, dst, count);
+    PVL_HIP(hipGetLastError());
+    return PVL_OK;
+}
+
 // Closed-body deflation: A[i][j] += c for i, j in [r0, r1) (rank-one shift c 1 1^T on the
 // body's block; removes the uniform-circulation null direction of a closed surface).
 __global__ void deflate_kernel(double *A, int n, int r0, int r1, double c)
