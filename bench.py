@@ -265,10 +265,9 @@ def run_c4(do_cpu: bool) -> dict:
     asm_ms, _, asm_pairs = prof_get(lib, "induction_matrix3")
     lib.pvl_prof_enable(0)
     diag = np.empty(n)
-    # deflation constant c = mean |diag A| (host copy of the diagonal)
+    # deflation constant c = mean |diag A| / n (host copy of the diagonal)
     Ah = A.numpy()
-    diag = np.diag(Ah).copy()
-    c = float(np.mean(np.abs(diag)))
+    c = LA.deflation_constant(np.diag(Ah))
     LA.deflate(A, 0, n, c)
     dirs = np.stack([G.cone_directions(1, 10.0, seed)[0] for seed in range(1, 17)])
     nh = nrm.numpy()

@@ -227,7 +227,7 @@ int pvl_mlp_forward(pvl_mlp *h, const float *X_dev, float *Y_dev, int batch);
 /* Gradient of the mean squared error against T (no update). */
 int pvl_mlp_gradient(pvl_mlp *h, const float *X_dev, const float *T_dev, int batch, float *grads_host,
                      float *loss_host);
-/* One Adam step with decoupled weight decay. */
+/* One Adam step (L2 weight decay wd * p added to the gradient); loss_host may be NULL. */
 int pvl_mlp_train_step(pvl_mlp *h, const float *X_dev, const float *T_dev, int batch, float lr, float beta1,
                        float beta2, float eps, float wd, float *loss_host);
 /* FP32 GMRES(m) (right Jacobi preconditioning) inside FP64 iterative refinement, starting

@@ -14,6 +14,8 @@
 // basis vector).
 #include "pvl_internal.hpp"
 
+#include <algorithm>
+#include <cfloat>
 #include <cmath>
 #include <vector>
 
@@ -283,7 +285,8 @@ __global__ void add_scaled_kernel(double *__restrict__ x, const float *__restric
         const double di = (double)d[i] * scale;
         const double xi = x[i] + di;
         x[i] = xi;
-        md = fmax(md, fabs(di));
+        // a NaN correction counts as infinite, so it can never pass for convergence
+        md = fabs(di) <= DBL_MAX ? fmax(md, fabs(di)) : INFINITY;
         mx = fmax(mx, fabs(xi));
     }
     pd[threadIdx.x] = md;
@@ -311,7 +314,7 @@ __global__ void maxabs_kernel(const double *__restrict__ r, int n, double *__res
     __shared__ double p[256];
     double m = 0.0;
     for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256)
-        m = fmax(m, fabs(r[i]));
+        m = fabs(r[i]) <= DBL_MAX ? fmax(m, fabs(r[i])) : INFINITY; // NaN or Inf -> Inf
     p[threadIdx.x] = m;
     __syncthreads();
     for (int w = 128; w > 0; w >>= 1)
@@ -344,12 +347,14 @@ int ensure_gmres_ws(int n, int m)
     }
     if (n > g_gw.n_cap || m > g_gw.m_cap)
     {
-        hipStreamSynchronize(stream());
-        hipFree(g_gw.V);
-        hipFree(g_gw.r32);
-        hipFree(g_gw.d32);
-        hipFree(g_gw.dinv);
-        hipFree(g_gw.r64);
+        n = std::max(n, g_gw.n_cap);
+        m = std::max(m, g_gw.m_cap);
+        (void)hipStreamSynchronize(stream());
+        (void)hipFree(g_gw.V);
+        (void)hipFree(g_gw.r32);
+        (void)hipFree(g_gw.d32);
+        (void)hipFree(g_gw.dinv);
+        (void)hipFree(g_gw.r64);
         PVL_HIP(hipMalloc(&g_gw.V, sizeof(float) * (size_t)(m + 1) * n));
         PVL_HIP(hipMalloc(&g_gw.r32, sizeof(float) * n));
         PVL_HIP(hipMalloc(&g_gw.d32, sizeof(float) * n));
@@ -411,6 +416,11 @@ int pvl_gmres_refine(int n, const double *A64_dev, const float *A32_dev, const d
             ratio = 0.0;
             break;
         }
+        if (!(rmax <= DBL_MAX))
+        {
+            ratio = INFINITY; // non-finite start or residual: report, the caller falls back
+            break;
+        }
         const double scale = 1.0 / rmax;
         r64_to_32_kernel<<<(n + 255) / 256, 256, 0, stream()>>>(g_gw.r64, g_gw.r32, n, scale);
         PVL_HIP(hipGetLastError());
@@ -450,6 +460,11 @@ int pvl_gmres_refine(int n, const double *A64_dev, const float *A32_dev, const d
         if (ratio <= tol)
         {
             converged = true;
+            ++outer;
+            break;
+        }
+        if (!(ratio <= DBL_MAX))
+        {
             ++outer;
             break;
         }

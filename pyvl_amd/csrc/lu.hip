@@ -17,6 +17,8 @@
 // diagonal blocks precomputed once per factorisation.
 #include "pvl_internal.hpp"
 
+#include <cfloat>
+
 #include <climits>
 #include <cmath>
 #include <vector>
@@ -598,7 +600,8 @@ __global__ void __launch_bounds__(256) colmax_kernel(const double *__restrict__ 
     double md = 0.0, mx = 0.0;
     for (int i = threadIdx.x; i < n; i += 256)
     {
-        md = fmax(md, fabs(D[(size_t)i * nrhs + q]));
+        const double d = fabs(D[(size_t)i * nrhs + q]);
+        md = d <= DBL_MAX ? fmax(md, d) : INFINITY; // a NaN correction never passes as converged
         mx = fmax(mx, fabs(X[(size_t)i * nrhs + q]));
     }
     __shared__ double sd[256], sx[256];
@@ -878,14 +881,14 @@ int pvl_lu_destroy(pvl_lu *h)
 {
     if (!h)
         return PVL_OK;
-    hipStreamSynchronize(stream());
+    (void)hipStreamSynchronize(stream());
     if (h->own_lu)
-        hipFree(h->lu);
-    hipFree(h->piv);
-    hipFree(h->perm);
-    hipFree(h->dinvL);
-    hipFree(h->dinvU);
-    hipFree(h->work);
+        (void)hipFree(h->lu);
+    (void)hipFree(h->piv);
+    (void)hipFree(h->perm);
+    (void)hipFree(h->dinvL);
+    (void)hipFree(h->dinvU);
+    (void)hipFree(h->work);
     delete h;
     return PVL_OK;
 }
@@ -999,7 +1002,8 @@ int pvl_solve_refine(pvl_lu *h, const double *A_dev, const double *B_dev, double
         return PVL_ERR_NOMEM;
     PVL_HIP(hipMemsetAsync(X_dev, 0, sizeof(double) * cnt, stream()));
     std::vector<double> hn(2 * nrhs);
-    int it = 0;
+    int it = 0, stalls = 0;
+    double prev_worst = INFINITY;
     for (; it < max_iter; ++it)
     {
         if (int st = pvl_residual(n, A_dev, X_dev, B_dev, R, nrhs))
@@ -1015,6 +1019,7 @@ int pvl_solve_refine(pvl_lu *h, const double *A_dev, const double *B_dev, double
         PVL_HIP(hipMemcpyAsync(hn.data(), norms, sizeof(double) * 2 * nrhs, hipMemcpyDeviceToHost, stream()));
         PVL_HIP(hipStreamSynchronize(stream()));
         bool done = true;
+        double worst = 0.0;
         for (int q = 0; q < nrhs; ++q)
         {
             const double r = hn[nrhs + q] > 0 ? hn[q] / hn[nrhs + q] : hn[q];
@@ -1022,8 +1027,17 @@ int pvl_solve_refine(pvl_lu *h, const double *A_dev, const double *B_dev, double
                 res_out[q] = r;
             if (!(r <= tol))
                 done = false;
+            worst = r <= DBL_MAX ? std::max(worst, r) : INFINITY;
         }
         if (done)
+        {
+            ++it;
+            break;
+        }
+        // stalled (the correction no longer shrinks by half twice in a row) or non-finite
+        stalls = worst > 0.5 * prev_worst ? stalls + 1 : 0;
+        prev_worst = worst;
+        if (stalls >= 2 || !(worst <= DBL_MAX))
         {
             ++it;
             break;

@@ -197,14 +197,14 @@ int ensure_batch(pvl_mlp *h, int batch)
 {
     if (batch <= h->cap_batch)
         return PVL_OK;
-    hipStreamSynchronize(stream());
+    (void)hipStreamSynchronize(stream());
     for (auto p : h->d_act)
-        hipFree(p);
+        (void)hipFree(p);
     for (auto p : h->d_z)
-        hipFree(p);
-    hipFree(h->d_delta[0]);
-    hipFree(h->d_delta[1]);
-    hipFree(h->d_out);
+        (void)hipFree(p);
+    (void)hipFree(h->d_delta[0]);
+    (void)hipFree(h->d_delta[1]);
+    (void)hipFree(h->d_out);
     const int L = h->n_layers;
     h->d_act.assign(L, nullptr);
     h->d_z.assign(L, nullptr);
@@ -343,20 +343,20 @@ int pvl_mlp_destroy(pvl_mlp *h)
 {
     if (!h)
         return PVL_OK;
-    hipStreamSynchronize(stream());
-    hipFree(h->d_p);
-    hipFree(h->d_g);
-    hipFree(h->d_m);
-    hipFree(h->d_v);
-    hipFree(h->d_pb);
-    hipFree(h->d_loss);
+    (void)hipStreamSynchronize(stream());
+    (void)hipFree(h->d_p);
+    (void)hipFree(h->d_g);
+    (void)hipFree(h->d_m);
+    (void)hipFree(h->d_v);
+    (void)hipFree(h->d_pb);
+    (void)hipFree(h->d_loss);
     for (auto p : h->d_act)
-        hipFree(p);
+        (void)hipFree(p);
     for (auto p : h->d_z)
-        hipFree(p);
-    hipFree(h->d_delta[0]);
-    hipFree(h->d_delta[1]);
-    hipFree(h->d_out);
+        (void)hipFree(p);
+    (void)hipFree(h->d_delta[0]);
+    (void)hipFree(h->d_delta[1]);
+    (void)hipFree(h->d_out);
     delete h;
     return PVL_OK;
 }
@@ -410,6 +410,11 @@ int pvl_mlp_gradient(pvl_mlp *h, const float *X_dev, const float *T_dev, int bat
 {
     if (int st = ensure_init())
         return st;
+    if (batch <= 0)
+    {
+        set_error("mlp_gradient: batch %d", batch);
+        return PVL_ERR_ARG;
+    }
     if (int st = forward(h, X_dev, batch))
         return st;
     if (int st = backward(h, T_dev, batch))
@@ -422,13 +427,18 @@ int pvl_mlp_gradient(pvl_mlp *h, const float *X_dev, const float *T_dev, int bat
     return PVL_OK;
 }
 
-// One Adam step (decoupled weight decay wd) on a batch; loss_host (optional) receives the
-// batch loss before the update (synchronises only if requested).
+// One Adam step on a batch, with L2 weight decay (wd * p added to the gradient); loss_host
+// (optional) receives the batch loss before the update (synchronises only if requested).
 int pvl_mlp_train_step(pvl_mlp *h, const float *X_dev, const float *T_dev, int batch, float lr, float beta1,
                        float beta2, float eps, float wd, float *loss_host)
 {
     if (int st = ensure_init())
         return st;
+    if (batch <= 0)
+    {
+        set_error("mlp_train_step: batch %d", batch);
+        return PVL_ERR_ARG;
+    }
     ProfScope prof("mlp_train_step", (double)batch);
     if (int st = forward(h, X_dev, batch))
         return st;

@@ -90,8 +90,8 @@ void *scratch(int slot, size_t bytes)
     if (s.first)
     {
         // Make sure no queued kernel still reads the old buffer.
-        hipStreamSynchronize(g_stream);
-        hipFree(s.first);
+        (void)hipStreamSynchronize(g_stream);
+        (void)hipFree(s.first);
         s = {nullptr, 0};
     }
     void *p = nullptr;
@@ -134,7 +134,8 @@ static hipEvent_t take_event()
         return e;
     }
     hipEvent_t e = nullptr;
-    hipEventCreate(&e);
+    if (hipEventCreate(&e) != hipSuccess)
+        return nullptr;
     return e;
 }
 
@@ -142,9 +143,10 @@ static void resolve_pending()
 {
     for (auto &p : g_pending)
     {
-        hipEventSynchronize(p.ev1);
+        (void)hipEventSynchronize(p.ev1);
         float ms = 0;
-        hipEventElapsedTime(&ms, p.ev0, p.ev1);
+        if (hipEventElapsedTime(&ms, p.ev0, p.ev1) != hipSuccess)
+            ms = 0;
         auto &e = g_prof_tab[p.name];
         e.ms += ms;
         e.count += 1;
@@ -161,14 +163,26 @@ ProfScope::ProfScope(const char *n, double work_units) : name(n), work(work_unit
         return;
     ev0 = take_event();
     ev1 = take_event();
-    hipEventRecord(ev0, g_stream);
+    if (!ev0 || !ev1 || hipEventRecord(ev0, g_stream) != hipSuccess)
+    {
+        if (ev0)
+            g_event_pool.push_back(ev0);
+        if (ev1)
+            g_event_pool.push_back(ev1);
+        ev0 = ev1 = nullptr;
+    }
 }
 
 ProfScope::~ProfScope()
 {
     if (!ev0)
         return;
-    hipEventRecord(ev1, g_stream);
+    if (hipEventRecord(ev1, g_stream) != hipSuccess)
+    {
+        g_event_pool.push_back(ev0);
+        g_event_pool.push_back(ev1);
+        return;
+    }
     g_pending.push_back({name, ev0, ev1, work});
     if (g_pending.size() > 4096)
         resolve_pending();

@@ -287,12 +287,12 @@ int pvl_mesh_destroy(pvl_mesh *m)
 {
     if (!m)
         return PVL_OK;
-    hipStreamSynchronize(stream());
-    hipFree(m->d_lines);
-    hipFree(m->d_offsets);
-    hipFree(m->d_entries);
-    hipFree(m->d_line_offsets);
-    hipFree(m->d_line_surfs);
+    (void)hipStreamSynchronize(stream());
+    (void)hipFree(m->d_lines);
+    (void)hipFree(m->d_offsets);
+    (void)hipFree(m->d_entries);
+    (void)hipFree(m->d_line_offsets);
+    (void)hipFree(m->d_line_surfs);
     delete m;
     return PVL_OK;
 }

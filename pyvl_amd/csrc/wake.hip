@@ -307,17 +307,17 @@ int pvl_wake_destroy(pvl_wake *w)
 {
     if (!w)
         return PVL_OK;
-    hipStreamSynchronize(stream());
+    (void)hipStreamSynchronize(stream());
     pvl_mesh_destroy(w->full);
     pvl_mesh_destroy(w->row0);
-    hipFree(w->d_te_nodes);
-    hipFree(w->d_adj);
-    hipFree(w->d_upos);
-    hipFree(w->d_circ[0]);
-    hipFree(w->d_circ[1]);
-    hipFree(w->d_old);
-    hipFree(w->d_vel);
-    hipFree(w->d_circ_x);
+    (void)hipFree(w->d_te_nodes);
+    (void)hipFree(w->d_adj);
+    (void)hipFree(w->d_upos);
+    (void)hipFree(w->d_circ[0]);
+    (void)hipFree(w->d_circ[1]);
+    (void)hipFree(w->d_old);
+    (void)hipFree(w->d_vel);
+    (void)hipFree(w->d_circ_x);
     delete w;
     return PVL_OK;
 }

@@ -1,0 +1,292 @@
+"""Learned warm start for the per-step circulation solve.
+
+A 4-layer MLP (width 256, BF16 products with FP32 accumulation on MFMA, FP32 master weights,
+Adam) is trained on the GPU, by libpvl_hip.so's own forward/backward kernels, on
+(features, circulation) pairs that the solver itself generates.  At solve time its
+prediction is the starting point of FP32 GMRES inside FP64 iterative refinement
+(``pvl_gmres_refine``); a solve that does not converge falls back to the FP64 LU.  There is
+no reference counterpart: the reference factors every step with LAPACK
+(python/pyvl/solver.py:223-227).
+
+Per-panel features (``FEATURES``): the step's right-hand side before and after the wake's
+explicit correction (free stream + body motion, and the wake-induced normal velocity),
+their change since the previous step, the previous step's circulation, the panel's position
+on the wing and its normal, whether it borders the trailing edge, and the body's pitch
+angle, pitch rate and heave velocity.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+from .device import DeviceArray
+
+FEATURES = (
+    "rhs_body", "rhs_wake", "rhs_total", "d_rhs_total", "gamma_prev", "x_rel", "y_rel", "chord_frac", "n_x", "n_z",
+    "trailing_edge", "pitch", "pitch_rate", "heave_rate",
+)
+
+
+class MLP:
+    """Device MLP: d_in -> width x (n_layers - 1) -> 1 with SiLU activations."""
+
+    def __init__(self, d_in: int, width: int = 256, n_layers: int = 4, seed: int = 42):
+        h = C.c_void_p()
+        _lib.call("pvl_mlp_create", C.byref(h), C.c_int(d_in), C.c_int(width), C.c_int(n_layers))
+        self._h = h
+        self.d_in, self.width, self.n_layers = d_in, width, n_layers
+        self.dims = [d_in] + [width] * (n_layers - 1) + [1]
+        n = C.c_long()
+        _lib.call("pvl_mlp_num_params", h, C.byref(n))
+        self.n_params = n.value
+        self.set_params(self.init_params(seed))
+
+    def init_params(self, seed: int) -> np.ndarray:
+        """He-uniform weights (SiLU), zero biases; layout: per layer W (out x in) then b."""
+        rng = np.random.default_rng(seed)
+        parts = []
+        for l in range(self.n_layers):
+            fan_in, fan_out = self.dims[l], self.dims[l + 1]
+            lim = np.sqrt(6.0 / fan_in) if l < self.n_layers - 1 else np.sqrt(3.0 / fan_in)
+            parts.append(rng.uniform(-lim, lim, (fan_out, fan_in)).ravel())
+            parts.append(np.zeros(fan_out))
+        p = np.concatenate(parts).astype(np.float32)
+        assert p.size == self.n_params
+        return p
+
+    def set_params(self, p: np.ndarray) -> None:
+        p = np.ascontiguousarray(p, np.float32)
+        _lib.call("pvl_mlp_set_params", self._h, p.ctypes.data_as(C.c_void_p))
+
+    def get_params(self) -> np.ndarray:
+        p = np.empty(self.n_params, np.float32)
+        _lib.call("pvl_mlp_get_params", self._h, p.ctypes.data_as(C.c_void_p))
+        return p
+
+    def unpack(self, p: np.ndarray):
+        """[(W, b)] per layer from a flat parameter vector."""
+        out, off = [], 0
+        for l in range(self.n_layers):
+            fi, fo = self.dims[l], self.dims[l + 1]
+            w = p[off:off + fi * fo].reshape(fo, fi)
+            off += fi * fo
+            b = p[off:off + fo]
+            off += fo
+            out.append((w, b))
+        return out
+
+    def forward(self, x: DeviceArray, out: DeviceArray | None = None) -> DeviceArray:
+        y = out if out is not None else DeviceArray((x.shape[0],), np.float32)
+        _lib.call("pvl_mlp_forward", self._h, x.cptr, y.cptr, C.c_int(x.shape[0]))
+        return y
+
+    def gradient(self, x: DeviceArray, t: DeviceArray):
+        g = np.empty(self.n_params, np.float32)
+        loss = C.c_float()
+        _lib.call("pvl_mlp_gradient", self._h, x.cptr, t.cptr, C.c_int(x.shape[0]), g.ctypes.data_as(C.c_void_p),
+                  C.byref(loss))
+        return g, float(loss.value)
+
+    def train_step(self, x: DeviceArray, t: DeviceArray, lr: float, wd: float = 0.0, want_loss: bool = False):
+        loss = C.c_float()
+        _lib.call("pvl_mlp_train_step", self._h, x.cptr, t.cptr, C.c_int(x.shape[0]), C.c_float(lr),
+                  C.c_float(0.9), C.c_float(0.999), C.c_float(1e-8), C.c_float(wd),
+                  C.byref(loss) if want_loss else None)
+        return float(loss.value) if want_loss else None
+
+    def __del__(self):
+        try:
+            if getattr(self, "_h", None) is not None and _lib._LIB is not None:
+                _lib._LIB.pvl_mlp_destroy(self._h)
+                self._h = None
+        except Exception:
+            pass
+
+
+def gmres_refine(a64: DeviceArray, a32: DeviceArray, b: DeviceArray, x: DeviceArray, tol: float = 1e-12,
+                 m: int = 32, inner_tol: float = 1e-6, max_outer: int = 8):
+    """x <- solution of A x = b (x holds the start).  Returns (converged, corrections,
+    gmres_steps, final max|dx|/max|x|)."""
+    stats = (C.c_int * 3)()
+    ratio = C.c_double()
+    st = _lib.load().pvl_gmres_refine(C.c_int(a64.shape[0]), a64.cptr, a32.cptr, b.cptr, x.cptr, C.c_double(tol),
+                                      C.c_int(m), C.c_double(inner_tol), C.c_int(max_outer), stats, C.byref(ratio))
+    if st not in (_lib.PVL_OK, _lib.PVL_ERR_NOTCONV):
+        _lib.check(st, "pvl_gmres_refine")
+    return st == _lib.PVL_OK, stats[0], stats[1], ratio.value
+
+
+class FeatureBuilder:
+    """Per-panel features of the current step (host side; N x len(FEATURES) float32)."""
+
+    def __init__(self, solver, pos0: np.ndarray, te_surfaces: np.ndarray, chord_root: float, span: float):
+        self.solver = solver
+        cps0 = solver.mesh.surface_average_vec3(pos0)
+        self.x_rel = cps0[:, 0] / chord_root
+        self.y_rel = cps0[:, 1] / (0.5 * span)
+        le = np.zeros_like(self.x_rel)
+        # local chord fraction: position between the panel row's leading and trailing x
+        ys = np.round(cps0[:, 1], 9)
+        for y in np.unique(ys):
+            sel = ys == y
+            xs = cps0[sel, 0]
+            lo, hi = xs.min(), xs.max()
+            le[sel] = (xs - lo) / max(hi - lo, 1e-12)
+        self.chord_frac = le
+        self.te = np.zeros(cps0.shape[0])
+        self.te[te_surfaces] = 1.0
+        self.prev_total = None
+        self.prev_gamma = None
+
+    def build(self, rhs_body: np.ndarray, rhs_total: np.ndarray, normals: np.ndarray, kin: tuple) -> np.ndarray:
+        n = rhs_body.size
+        d_tot = rhs_total - (self.prev_total if self.prev_total is not None else rhs_total)
+        gp = self.prev_gamma if self.prev_gamma is not None else np.zeros(n)
+        pitch, pitch_rate, heave_rate = kin
+        f = np.stack([
+            rhs_body, rhs_total - rhs_body, rhs_total, d_tot, gp, self.x_rel, self.y_rel, self.chord_frac,
+            normals[:, 0], normals[:, 2], self.te, np.full(n, np.sin(pitch)), np.full(n, pitch_rate),
+            np.full(n, heave_rate),
+        ], axis=1)
+        return f.astype(np.float32)
+
+    def commit(self, rhs_total: np.ndarray, gamma: np.ndarray) -> None:
+        self.prev_total = rhs_total.copy()
+        self.prev_gamma = gamma.copy()
+
+
+class Normaliser:
+    """Feature and target standardisation from training statistics."""
+
+    def __init__(self, X: np.ndarray, y: np.ndarray):
+        self.mu = X.mean(axis=0)
+        self.sd = X.std(axis=0)
+        self.sd[self.sd < 1e-12] = 1.0
+        self.ymu = float(y.mean())
+        self.ysd = float(y.std()) or 1.0
+
+    def x(self, X: np.ndarray) -> np.ndarray:
+        return ((X - self.mu) / self.sd).astype(np.float32)
+
+    def y(self, y: np.ndarray) -> np.ndarray:
+        return ((y - self.ymu) / self.ysd).astype(np.float32)
+
+    def y_inv(self, z: np.ndarray) -> np.ndarray:
+        return z.astype(np.float64) * self.ysd + self.ymu
+
+
+# ---------------------------------------------------------------------------------------
+# The pitch + heave tapered-wing workload (2,048 panels, free wake): data, training, use.
+def _kinematics(frame, t):
+    th, thd, _, hd = frame._angles(t)
+    return float(th), float(thd), float(hd)
+
+
+def run_c5(kin_seed: int, steps: int, solve: str = "lu64", warm: str = "none", mlp: MLP | None = None,
+           norm: Normaliser | None = None, record: bool = False, rows: int | None = None):
+    """Run the workload; returns dict with per-step circulations, solve times and GMRES stats.
+
+    solve "lu64": FP64 LU every step; "gmres": FP32 GMRES in FP64 refinement (LU fallback)
+    started from zero ("none"), the previous step's circulation ("prev") or the MLP ("mlp").
+    """
+    import time
+
+    from .solver import C5, make_c5_solver
+
+    sol = make_c5_solver(kin_seed, solve=solve, rows=rows)
+    n = sol.mesh.n_surfaces
+    if sol.d_b0 is None:
+        sol.d_b0 = DeviceArray((n,))
+    adj = np.asarray(sol.wake.adjacent_surfaces).reshape(-1, 2)
+    te = np.unique(adj[(adj != 0x7FFFFFFF)].astype(np.int64))
+    fb = FeatureBuilder(sol, sol.p0, te, C5["chord"], C5["span"])
+    dX = DeviceArray((n, len(FEATURES)), np.float32)
+    dY = DeviceArray((n,), np.float32)
+    gammas, times, feats = [], [], []
+    x0 = {"v": None}
+    sol.warm_start = lambda s: x0["v"]
+    for it in range(steps):
+        t = it * sol.dt
+        sol.assemble(t)
+        _lib.call("pvl_synchronize")
+        t0 = time.perf_counter()
+        need_feats = record or warm == "mlp"
+        if need_feats:
+            rb, rt, nr = sol.d_b0.numpy(), sol.d_b.numpy(), sol.d_nrm.numpy()
+            f = fb.build(rb, rt, nr, _kinematics(sol.frame, t))
+        if warm == "mlp":
+            dX.copy_from(norm.x(f))
+            mlp.forward(dX, dY)
+            x0["v"] = norm.y_inv(dY.numpy())
+        elif warm == "prev":
+            x0["v"] = fb.prev_gamma
+        else:
+            x0["v"] = None
+        sol.solve_system()
+        _lib.call("pvl_synchronize")
+        times.append(time.perf_counter() - t0)
+        g = sol.d_x.numpy()
+        gammas.append(g)
+        if record:
+            feats.append(f)
+        if need_feats:
+            fb.commit(rt, g)
+        else:
+            fb.prev_gamma = g
+        sol.wake.update(t, sol.mesh, sol.d_pos, sol.d_x, sol.flow)
+        sol.iteration += 1
+    return {
+        "gamma": np.array(gammas), "solve_s": np.array(times), "features": np.array(feats) if record else None,
+        "gmres": list(sol.gmres_stats), "fallbacks": sol.fallbacks,
+    }
+
+
+def generate_training_data(n_steps: int = 2000, seeds=tuple(range(100, 108)), rows: int | None = None):
+    """Solver-generated (features, circulation) pairs from FP64-LU runs of the workload
+    with kinematics seeds disjoint from the evaluation seed."""
+    per = int(np.ceil(n_steps / len(seeds)))
+    X, y = [], []
+    left = n_steps
+    for s in seeds:
+        k = min(per, left)
+        if k <= 0:
+            break
+        r = run_c5(s, k, solve="lu64", record=True, rows=rows)
+        X.append(r["features"].reshape(-1, len(FEATURES)))
+        y.append(r["gamma"].reshape(-1))
+        left -= k
+    return np.concatenate(X), np.concatenate(y)
+
+
+def train(mlp: MLP, X: np.ndarray, y: np.ndarray, norm: Normaliser, epochs: int = 4, batch: int = 8192,
+          lr: float = 2e-3, seed: int = 42, log=None):
+    """Adam on shuffled mini-batches, cosine learning-rate decay; returns per-epoch losses
+    (mean squared error of the standardised target)."""
+    rng = np.random.default_rng(seed)
+    Xn, yn = norm.x(X), norm.y(y)
+    n = Xn.shape[0] // batch * batch
+    dX = DeviceArray((n, Xn.shape[1]), np.float32)
+    dy = DeviceArray((n,), np.float32)
+    total = epochs * (n // batch)
+    k = 0
+    losses = []
+    for ep in range(epochs):
+        perm = rng.permutation(Xn.shape[0])[:n]
+        dX.copy_from(Xn[perm])
+        dy.copy_from(yn[perm])
+        ep_loss = []
+        for b in range(n // batch):
+            cur = lr * 0.5 * (1.0 + np.cos(np.pi * k / total))
+            want = b % 50 == 0 or b == n // batch - 1
+            l = mlp.train_step(dX.view(b * batch * Xn.shape[1], (batch, Xn.shape[1])), dy.view(b * batch, (batch,)),
+                               cur, wd=1e-5, want_loss=want)
+            if want:
+                ep_loss.append(l)
+            k += 1
+        losses.append(float(np.mean(ep_loss)))
+        if log:
+            log(f"epoch {ep}: loss {losses[-1]:.3e}")
+    return losses

@@ -165,10 +165,18 @@ class UnsteadySolver:
         self.d_A = DeviceArray((n, n))
         self.d_b = DeviceArray((n,))
         self.d_x = DeviceArray((n,))
-        self.lu = LA.LU(n, LA.FP64 if solve == "lu64" else LA.FP32)
+        self.lu = LA.LU(n, LA.FP32 if solve == "ir32" else LA.FP64)
         self.vinf = np.ascontiguousarray(flow.vector)
         self.iteration = 0
         self.ir_iters: list[int] = []
+        # "gmres": FP32 GMRES in FP64 refinement from warm_start(self) (numpy (n,) or None
+        # for zero), LU64 fallback when it does not converge
+        self.warm_start = None
+        self.gmres_tol = 1e-12
+        self.gmres_stats: list[tuple] = []
+        self.fallbacks = 0
+        self.d_b0 = DeviceArray((n,)) if solve == "gmres" else None
+        self.d_A32 = DeviceArray((n, n), np.float32) if solve == "gmres" else None
 
     def _kinematics(self, t: float) -> None:
         self.d_pos.copy_from(self.frame.positions(self.p0, t))
@@ -184,12 +192,36 @@ class UnsteadySolver:
         _lib.call("pvl_rhs_normal_velocity", C.c_uint(n), self.d_nrm.cptr, self.d_cpv.cptr,
                   self.vinf.ctypes.data_as(C.POINTER(C.c_double)), self.d_b.cptr)
         self.mesh.induction_matrix3(self.tol, self.d_pos, self.d_cp, self.d_nrm, self.d_A, precision=self.prec)
+        if self.d_b0 is not None:
+            _lib.call("pvl_memcpy_d2d", self.d_b0.cptr, self.d_b.cptr, C.c_size_t(8 * n))
         if self.wake is not None and self.iteration > 0:
             self.wake.apply_corrections(self.d_cp, self.d_nrm, self.d_A, self.d_b)
 
+    def _lu64(self) -> None:
+        n = self.mesh.n_surfaces
+        self.lu.factor(self.d_A)
+        _lib.call("pvl_memcpy_d2d", self.d_x.cptr, self.d_b.cptr, C.c_size_t(8 * n))
+        self.lu.solve(self.d_x)
+
     def solve_system(self) -> None:
         n = self.mesh.n_surfaces
-        if self.solve == "lu64":
+        if self.solve == "gmres":
+            from .learned import gmres_refine
+
+            _lib.call("pvl_convert_f64_f32", self.d_A.cptr, self.d_A32.cptr, C.c_size_t(n * n))
+            x0 = self.warm_start(self) if self.warm_start is not None else None
+            if x0 is None:
+                _lib.call("pvl_memset_zero", self.d_x.cptr, C.c_size_t(8 * n))
+            elif isinstance(x0, DeviceArray):
+                _lib.call("pvl_memcpy_d2d", self.d_x.cptr, x0.cptr, C.c_size_t(8 * n))
+            else:
+                self.d_x.copy_from(np.ascontiguousarray(x0, np.float64))
+            ok, outer, steps, ratio = gmres_refine(self.d_A, self.d_A32, self.d_b, self.d_x, tol=self.gmres_tol)
+            self.gmres_stats.append((outer, steps, ratio, ok))
+            if not ok:
+                self.fallbacks += 1
+                self._lu64()
+        elif self.solve == "lu64":
             self.lu.factor(self.d_A)
             _lib.call("pvl_memcpy_d2d", self.d_x.cptr, self.d_b.cptr, C.c_size_t(8 * n))
             self.lu.solve(self.d_x)
@@ -224,7 +256,7 @@ class UnsteadySolver:
 
 def make_wing_wake_solver(n_span: int, n_chord: int, span: float, chord: float, frame, flow, line_rows: int,
                           dt: float, *, vortex_limit: float = 1e-6, taper: float = 1.0, mode: str = "reference",
-                          induction_precision: int = 0, solve: str = "lu64"):
+                          induction_precision: int = 0, solve: str = "lu64") -> "UnsteadySolver":
     """Flat (optionally tapered) wing shedding from its trailing edge, as in
     examples/plot_example_2.py:51-90 with the wake model of python/pyvl/wake_models.py."""
     from .geometry import flat_wing
@@ -245,3 +277,26 @@ def make_wing_wake_solver(n_span: int, n_chord: int, span: float, chord: float, 
     solver = UnsteadySolver(mesh, pos0, frame, flow, vortex_limit, dt, wake, induction_precision=induction_precision,
                             solve=solve, closed=False)
     return solver
+
+
+def c5_kinematics(seed: int) -> "PitchHeaveFrame":
+    """Seeded random pitch + heave: pitch amplitude U(0, 10 deg), heave amplitude U(0, 0.2)
+    chord, reduced frequency k U(0.05, 0.5) (omega = 2 k U / c with U = c = 1), phases
+    U(0, 2 pi), pitch axis at quarter chord."""
+    rng = np.random.default_rng(seed)
+    theta0 = np.radians(rng.uniform(0.0, 10.0))
+    h0 = rng.uniform(0.0, 0.2)
+    k = rng.uniform(0.05, 0.5)
+    return PitchHeaveFrame(theta0, h0, 2.0 * k, 0.25, rng.uniform(0, 2 * np.pi), rng.uniform(0, 2 * np.pi))
+
+
+C5 = dict(n_span=64, n_chord=32, span=8.0, chord=1.0, taper=0.5, rows=300, dt=1.0 / 32.0, tol=1e-6)
+
+
+def make_c5_solver(kin_seed: int, solve: str = "lu64", mode: str = "free", rows: int | None = None):
+    """Pitching + heaving tapered wing, 64 x 32 = 2,048 panels (span 8, root chord 1, taper
+    0.5), free wake of ``rows`` rows (300 by default), FP64 induction."""
+    fc = FlowConditionsUniform(1.0, 0.0, 0.0)
+    return make_wing_wake_solver(C5["n_span"], C5["n_chord"], C5["span"], C5["chord"], c5_kinematics(kin_seed), fc,
+                                 rows or C5["rows"], C5["dt"], vortex_limit=C5["tol"], taper=C5["taper"], mode=mode,
+                                 solve=solve)

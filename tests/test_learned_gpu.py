@@ -1,0 +1,182 @@
+"""Learned warm start on the GPU: the MLP kernels against the CPU oracle, GMRES refinement
+against the FP64 LU, and what the trained network gains over the method without it.
+
+Tolerances (BF16 operands, FP32 accumulation): network output within 1e-2 relative
+(Frobenius) of the oracle that rounds at the same points, gradients within 2e-2 per layer
+of it and within 6e-2 of the exact float64 gradient; Adam within 1e-5 * lr of its formula
+applied to the device gradient.  Solves: 1e-10 relative to the FP64 LU.
+"""
+
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from oracle import mlp_oracle as MO
+from pyvl_amd import DeviceArray, _lib
+from pyvl_amd import geometry as G
+from pyvl_amd import learned as L
+from pyvl_amd import linalg as LA
+from pyvl_amd.cvl import Mesh
+
+pytestmark = pytest.mark.gpu
+
+DIMS = (14, 256, 256, 256, 1)
+
+
+def _rel(a, b):
+    return float(np.linalg.norm(np.asarray(a, np.float64) - b) / np.linalg.norm(b))
+
+
+def _net(seed=1):
+    net = L.MLP(14, 256, 4, seed=seed)
+    p = net.get_params().astype(np.float64)
+    # non-zero biases so the bias paths are exercised
+    rng = np.random.default_rng(seed)
+    for w, b in net.unpack(p):
+        b[:] = rng.normal(0.0, 0.1, b.size)
+    net.set_params(p.astype(np.float32))
+    return net, net.get_params().astype(np.float64)
+
+
+def _data(n, seed):
+    rng = np.random.default_rng(seed)
+    X = rng.normal(size=(n, 14)).astype(np.float32)
+    t = (np.sin(X[:, 0]) + X[:, 1] * X[:, 2] - 0.5 * X[:, 3]).astype(np.float32)
+    return X, t
+
+
+@pytest.mark.parametrize("batch", [1, 63, 300, 2048])
+def test_forward_matches_oracle(batch):
+    net, p = _net()
+    X, _ = _data(batch, batch)
+    y = net.forward(DeviceArray.from_numpy(X)).numpy()
+    y_ref, _, _ = MO.forward(p, DIMS, X, bf16=True)
+    assert _rel(y, y_ref) < 1e-2
+    y_exact, _, _ = MO.forward(p, DIMS, X)
+    assert _rel(y, y_exact) < 3e-2
+
+
+@pytest.mark.parametrize("batch", [64, 777])
+def test_gradient_matches_oracle(batch):
+    net, p = _net(3)
+    X, t = _data(batch, 10 + batch)
+    g, loss = net.gradient(DeviceArray.from_numpy(X), DeviceArray.from_numpy(t))
+    loss_ref, g_ref = MO.loss_and_gradient(p, DIMS, X, t, bf16=True)
+    _, g_exact = MO.loss_and_gradient(p, DIMS, X, t)
+    assert abs(loss - loss_ref) <= 1e-2 * loss_ref
+    off = 0
+    for l in range(len(DIMS) - 1):
+        nw = DIMS[l] * DIMS[l + 1]
+        for sl in (slice(off, off + nw), slice(off + nw, off + nw + DIMS[l + 1])):
+            assert _rel(g[sl], g_ref[sl]) < 2e-2, (l, sl)
+            assert _rel(g[sl], g_exact[sl]) < 6e-2, (l, sl)
+        off += nw + DIMS[l + 1]
+    assert off == g.size
+
+
+def test_adam_step_matches_oracle():
+    net, p = _net(5)
+    X, t = _data(512, 5)
+    dX, dt = DeviceArray.from_numpy(X), DeviceArray.from_numpy(t)
+    g, _ = net.gradient(dX, dt)
+    lr, wd = 1e-3, 1e-4
+    net.train_step(dX, dt, lr, wd=wd)
+    p1 = net.get_params()
+    p_ref, _, _ = MO.adam(p, g.astype(np.float64), np.zeros_like(p), np.zeros_like(p), 1, lr, wd=wd)
+    assert np.max(np.abs(p1 - p_ref)) <= 1e-5 * lr + 1e-7 * np.max(np.abs(p))
+
+
+def test_training_reduces_loss():
+    net = L.MLP(14, 256, 4, seed=11)
+    X, t = _data(8192, 11)
+    dX, dt = DeviceArray.from_numpy(X), DeviceArray.from_numpy(t)
+    first = net.train_step(dX, dt, 1e-3, want_loss=True)
+    for _ in range(150):
+        last = net.train_step(dX, dt, 1e-3, want_loss=True)
+    assert last < 0.1 * first
+
+
+def test_bad_batch_is_an_error():
+    net = L.MLP(14, 64, 3)
+    x = DeviceArray((1, 14), np.float32)
+    with pytest.raises(_lib.PvlError):
+        net.gradient(x.view(0, (0, 14)), DeviceArray((1,), np.float32).view(0, (0,)))
+
+
+# ---------------------------------------------------------------------------------------
+def _wing_system(n_span=32, n_chord=8):
+    pos, elements, _ = G.flat_wing(n_span, n_chord, 8.0, 1.0, taper=0.5)
+    m = Mesh(pos.shape[0], elements)
+    aoa = np.radians(5.0)
+    p = pos.copy()
+    p[:, 2] = -np.sin(aoa) * p[:, 0]
+    p[:, 0] = np.cos(aoa) * p[:, 0]
+    cps = m.surface_average_vec3(p)
+    nrm = m.surface_normal(p)
+    A = m.induction_matrix3(1e-6, p, cps, nrm)
+    b = -nrm @ np.array([1.0, 0.0, 0.0])
+    return A, b
+
+
+def _gmres(A, b, x0=None, **kw):
+    n = b.size
+    dA = DeviceArray.from_numpy(A)
+    dA32 = DeviceArray((n, n), np.float32)
+    _lib.call("pvl_convert_f64_f32", dA.cptr, dA32.cptr, C.c_size_t(n * n))
+    dx = DeviceArray.from_numpy(np.zeros(n) if x0 is None else x0)
+    ok, outer, steps, ratio = L.gmres_refine(dA, dA32, DeviceArray.from_numpy(b), dx, **kw)
+    return ok, outer, steps, ratio, dx.numpy()
+
+
+def test_gmres_wing_matches_lu():
+    A, b = _wing_system()
+    x_ref = LA.LU(b.size).factor(A).solve(b)
+    ok, outer, steps, ratio, x = _gmres(A, b)
+    assert ok and ratio <= 1e-12
+    assert _rel(x, x_ref) < 1e-10
+    # started at the answer: at most one correction, no more GMRES steps than a cold start
+    ok2, outer2, steps2, _, x2 = _gmres(A, b, x_ref)
+    assert ok2 and outer2 <= 1 and steps2 <= steps
+    assert _rel(x2, x_ref) < 1e-10
+
+
+@pytest.mark.parametrize("n", [1, 100, 1000, 1537])
+def test_gmres_random_matches_lu(n):
+    rng = np.random.default_rng(n)
+    A = rng.normal(size=(n, n)) / np.sqrt(n) + 2.0 * np.eye(n) * (1.0 + rng.uniform(size=n))
+    b = rng.normal(size=n)
+    x_ref = LA.LU(n).factor(A).solve(b)
+    ok, _, _, _, x = _gmres(A, b)
+    assert ok
+    assert _rel(x, x_ref) < 1e-10
+
+
+def test_gmres_reports_non_convergence():
+    """A budget too small to converge returns not-converged, which the solver turns into the
+    LU fallback."""
+    A, b = _wing_system()
+    ok, outer, steps, ratio, _ = _gmres(A, b, m=2, max_outer=1)
+    assert not ok and outer == 1 and ratio > 1e-12
+
+
+def test_learned_warm_start_gains():
+    """Reduced pitch + heave workload (2,048 panels, 40 steps, 40-row free wake): a network
+    trained on two other kinematics seeds makes GMRES converge in fewer steps than the
+    zero start, and every step's circulation stays within 1e-10 of the FP64 LU run."""
+    rows, steps = 40, 40
+    X, y = L.generate_training_data(n_steps=2 * steps, seeds=(100, 101), rows=rows)
+    norm = L.Normaliser(X, y)
+    net = L.MLP(X.shape[1], 256, 4, seed=42)
+    losses = L.train(net, X, y, norm, epochs=6, batch=4096, lr=2e-3)
+    assert losses[-1] < 0.2 * losses[0]
+    ref = L.run_c5(7, steps, solve="lu64", rows=rows)
+    cold = L.run_c5(7, steps, solve="gmres", warm="none", rows=rows)
+    warm = L.run_c5(7, steps, solve="gmres", warm="mlp", mlp=net, norm=norm, rows=rows)
+    for r in (cold, warm):
+        assert r["fallbacks"] == 0
+        err = np.max(np.abs(r["gamma"] - ref["gamma"]), axis=1) / np.max(np.abs(ref["gamma"]), axis=1)
+        assert np.max(err) < 1e-10
+    s_cold = np.mean([s[1] for s in cold["gmres"]])
+    s_warm = np.mean([s[1] for s in warm["gmres"]])
+    assert s_warm < 0.8 * s_cold, (s_warm, s_cold)

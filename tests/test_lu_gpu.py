@@ -135,7 +135,7 @@ def test_closed_body_deflated_solve():
     a = O.induction_matrix3(t, pos, c, n, 1e-6)
     b = np.sum(n * -np.array([1.0, 0.0, 0.0])[None, :], axis=1)
     sols = []
-    for shift in (np.mean(np.abs(np.diag(a))), 1.0):
+    for shift in (LA.deflation_constant(np.diag(a)), np.mean(np.abs(np.diag(a))), 1.0):
         da = DeviceArray.from_numpy(a)
         LA.deflate(da, 0, a.shape[0], shift)
         ad = da.numpy()
@@ -147,3 +147,4 @@ def test_closed_body_deflated_solve():
         assert _rel(x, x_ref) < 1e-12
         sols.append(x)
     assert _rel(sols[0], sols[1]) < 1e-10
+    assert _rel(sols[0], sols[2]) < 1e-10
