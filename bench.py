@@ -333,6 +333,46 @@ def run_c4(do_cpu: bool) -> dict:
     return out
 
 
+def run_c5(train_steps: int = 2000, eval_steps: int = 300, seed: int = 7) -> dict:
+    """Learned warm start: pitch + heave tapered wing, 2,048 panels, 300-row free wake.
+
+    The network is trained on ``train_steps`` FP64-LU steps from kinematics seeds 100-107
+    (disjoint from the evaluation seed), then the evaluation run is repeated with the plain
+    FP64 LU and with FP32 GMRES in FP64 refinement started from zero, from the previous
+    step's circulation and from the network's prediction.  Errors: per step, max|dG| /
+    max|G| against the FP64 LU run."""
+    from pyvl_amd import learned as L
+
+    t0 = time.perf_counter()
+    X, y = L.generate_training_data(train_steps, seeds=tuple(range(100, 108)))
+    t_gen = time.perf_counter() - t0
+    norm = L.Normaliser(X, y)
+    net = L.MLP(X.shape[1], 256, 4, seed=42)
+    t0 = time.perf_counter()
+    losses = L.train(net, X, y, norm, epochs=4, batch=8192, lr=2e-3, log=log)
+    t_train = time.perf_counter() - t0
+    log(f"C5 data: {X.shape[0]} samples in {t_gen:.1f} s; training {t_train:.1f} s, losses {losses}")
+    runs = {"lu64": L.run_c5(seed, eval_steps, solve="lu64")}
+    for w in ("none", "prev", "mlp"):
+        runs["gmres_" + w] = L.run_c5(seed, eval_steps, solve="gmres", warm=w, mlp=net, norm=norm)
+    ref = runs["lu64"]["gamma"]
+    out = {"panels": int(ref.shape[1]), "train_samples": int(X.shape[0]), "train_data_s": round(t_gen, 2),
+           "train_s": round(t_train, 2), "train_loss_first_last": [losses[0], losses[-1]]}
+    for k, r in runs.items():
+        err = np.max(np.abs(r["gamma"] - ref), axis=1) / np.max(np.abs(ref), axis=1)
+        st = r["gmres"]
+        out[k] = {
+            "run_s": round(float(np.sum(r["step_s"])), 4),
+            "solve_ms_mean": round(1e3 * float(np.mean(r["solve_s"])), 4),
+            "gmres_steps_mean": round(float(np.mean([s[1] for s in st])), 3) if st else None,
+            "corrections_mean": round(float(np.mean([s[0] for s in st])), 3) if st else None,
+            "fallbacks": r["fallbacks"],
+            "max_rel_err_vs_fp64_lu": float(np.max(err)),
+        }
+        log(f"C5 {k}: {out[k]}")
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -361,6 +401,7 @@ def main() -> int:
         extra["c3_reference_semantics_steps_per_s"] = round(ref["steps_per_s"], 3)
         extra["c3_reference_semantics_grow_400_s"] = round(ref["grow_400_s"], 3)
         extra["solve_16k"] = run_c4(not args.no_cpu)
+        extra["learned_warm_start"] = run_c5()
         if not args.no_cpu:
             extra["cpu_reference_semantics"] = cpu_reference_step(ref, 2)
             log(f"CPU reference semantics: {extra['cpu_reference_semantics']}")

@@ -1,9 +1,10 @@
 """Learned warm start for the per-step circulation solve.
 
 A 4-layer MLP (width 256, BF16 products with FP32 accumulation on MFMA, FP32 master weights,
-Adam) is trained on the GPU, by libpvl_hip.so's own forward/backward kernels, on
-(features, circulation) pairs that the solver itself generates.  At solve time its
-prediction is the starting point of FP32 GMRES inside FP64 iterative refinement
+Adam) is trained on the GPU, by libpvl_hip.so's own forward/backward kernels, on pairs
+that the solver itself generates: per panel, the step's features and the change of its
+circulation since the previous step.  At solve time the previous circulation plus the
+predicted change is the starting point of FP32 GMRES inside FP64 iterative refinement
 (``pvl_gmres_refine``); a solve that does not converge falls back to the FP64 LU.  There is
 no reference counterpart: the reference factors every step with LAPACK
 (python/pyvl/solver.py:223-227).
@@ -186,16 +187,21 @@ def _kinematics(frame, t):
 
 
 def run_c5(kin_seed: int, steps: int, solve: str = "lu64", warm: str = "none", mlp: MLP | None = None,
-           norm: Normaliser | None = None, record: bool = False, rows: int | None = None):
-    """Run the workload; returns dict with per-step circulations, solve times and GMRES stats.
+           norm: Normaliser | None = None, record: bool = False, rows: int | None = None, pre_solve=None):
+    """Run the workload; returns per-step circulations, times and GMRES statistics.
 
     solve "lu64": FP64 LU every step; "gmres": FP32 GMRES in FP64 refinement (LU fallback)
-    started from zero ("none"), the previous step's circulation ("prev") or the MLP ("mlp").
+    started from zero ("none"), the previous step's circulation ("prev") or the previous
+    circulation plus the network's predicted change ("mlp"; zero start at step 0, which has
+    no previous circulation).  ``pre_solve(solver, it)`` is called after the step's system
+    is assembled.  ``solve_s``: features + prediction + solve; ``step_s``: the whole step.
     """
     import time
 
     from .solver import C5, make_c5_solver
 
+    if warm == "mlp" and (mlp is None or norm is None):
+        raise ValueError("warm='mlp' needs a trained network and its normaliser")
     sol = make_c5_solver(kin_seed, solve=solve, rows=rows)
     n = sol.mesh.n_surfaces
     if sol.d_b0 is None:
@@ -205,23 +211,27 @@ def run_c5(kin_seed: int, steps: int, solve: str = "lu64", warm: str = "none", m
     fb = FeatureBuilder(sol, sol.p0, te, C5["chord"], C5["span"])
     dX = DeviceArray((n, len(FEATURES)), np.float32)
     dY = DeviceArray((n,), np.float32)
-    gammas, times, feats = [], [], []
+    gammas, times, steps_s, feats = [], [], [], []
     x0 = {"v": None}
     sol.warm_start = lambda s: x0["v"]
+    need_feats = record or warm == "mlp"
+    _lib.call("pvl_synchronize")
     for it in range(steps):
+        ts = time.perf_counter()
         t = it * sol.dt
         sol.assemble(t)
+        if pre_solve is not None:
+            pre_solve(sol, it)
         _lib.call("pvl_synchronize")
         t0 = time.perf_counter()
-        need_feats = record or warm == "mlp"
         if need_feats:
             rb, rt, nr = sol.d_b0.numpy(), sol.d_b.numpy(), sol.d_nrm.numpy()
             f = fb.build(rb, rt, nr, _kinematics(sol.frame, t))
-        if warm == "mlp":
+        if warm == "mlp" and fb.prev_gamma is not None:
             dX.copy_from(norm.x(f))
             mlp.forward(dX, dY)
-            x0["v"] = norm.y_inv(dY.numpy())
-        elif warm == "prev":
+            x0["v"] = fb.prev_gamma + norm.y_inv(dY.numpy())
+        elif warm in ("prev", "mlp"):
             x0["v"] = fb.prev_gamma
         else:
             x0["v"] = None
@@ -238,15 +248,19 @@ def run_c5(kin_seed: int, steps: int, solve: str = "lu64", warm: str = "none", m
             fb.prev_gamma = g
         sol.wake.update(t, sol.mesh, sol.d_pos, sol.d_x, sol.flow)
         sol.iteration += 1
+        _lib.call("pvl_synchronize")
+        steps_s.append(time.perf_counter() - ts)
     return {
-        "gamma": np.array(gammas), "solve_s": np.array(times), "features": np.array(feats) if record else None,
-        "gmres": list(sol.gmres_stats), "fallbacks": sol.fallbacks,
+        "gamma": np.array(gammas), "solve_s": np.array(times), "step_s": np.array(steps_s),
+        "features": np.array(feats) if record else None, "gmres": list(sol.gmres_stats), "fallbacks": sol.fallbacks,
     }
 
 
 def generate_training_data(n_steps: int = 2000, seeds=tuple(range(100, 108)), rows: int | None = None):
-    """Solver-generated (features, circulation) pairs from FP64-LU runs of the workload
-    with kinematics seeds disjoint from the evaluation seed."""
+    """Solver-generated training pairs from FP64-LU runs of the workload, with kinematics
+    seeds disjoint from the evaluation seed: per panel and step, the features and the
+    change of circulation since the previous step (step 0, without a previous circulation,
+    is left out)."""
     per = int(np.ceil(n_steps / len(seeds)))
     X, y = [], []
     left = n_steps
@@ -255,8 +269,9 @@ def generate_training_data(n_steps: int = 2000, seeds=tuple(range(100, 108)), ro
         if k <= 0:
             break
         r = run_c5(s, k, solve="lu64", record=True, rows=rows)
-        X.append(r["features"].reshape(-1, len(FEATURES)))
-        y.append(r["gamma"].reshape(-1))
+        g = r["gamma"]
+        X.append(r["features"][1:].reshape(-1, len(FEATURES)))
+        y.append((g[1:] - g[:-1]).reshape(-1))
         left -= k
     return np.concatenate(X), np.concatenate(y)
 
