@@ -230,11 +230,14 @@ int pvl_mlp_gradient(pvl_mlp *h, const float *X_dev, const float *T_dev, int bat
 /* One Adam step (L2 weight decay wd * p added to the gradient); loss_host may be NULL. */
 int pvl_mlp_train_step(pvl_mlp *h, const float *X_dev, const float *T_dev, int batch, float lr, float beta1,
                        float beta2, float eps, float wd, float *loss_host);
-/* FP32 GMRES(m) (right Jacobi preconditioning) inside FP64 iterative refinement, starting
- * from x_dev.  stats[0] = corrections, stats[1] = GMRES steps, stats[2] = converged flag.
- * PVL_ERR_NOTCONV when max|dx| <= tol * max|x| was not reached within max_outer. */
+/* Iterative solve of A x = b from the start x_dev: FP64 residuals r = b - A x, stop when
+ * max|r| <= tol * max|b|, else one FP32 GMRES(m) cycle (right Jacobi preconditioning) on
+ * A d = r, x += d, at most max_outer corrections.  stats[0] = corrections, stats[1] = GMRES
+ * steps, stats[2] = converged flag; res_out[0] / res_out[1] = max|r|/max|b| at the start /
+ * at the end.  PVL_ERR_NOTCONV when not converged (the caller falls back to LU).  No
+ * reference counterpart (the reference factors every step, python/pyvl/solver.py:223-227). */
 int pvl_gmres_refine(int n, const double *A64_dev, const float *A32_dev, const double *b_dev, double *x_dev, double tol,
-                     int m, double inner_tol, int max_outer, int *stats, double *final_ratio);
+                     int m, double inner_tol, int max_outer, int *stats, double *res_out);
 /* FP32 copy of an FP64 device array. */
 int pvl_convert_f64_f32(const double *src_dev, float *dst_dev, size_t count);
 

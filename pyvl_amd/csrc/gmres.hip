@@ -275,38 +275,10 @@ __global__ void r64_to_32_kernel(const double *__restrict__ r, float *__restrict
         r32[i] = (float)(r[i] * scale);
 }
 
-__global__ void add_scaled_kernel(double *__restrict__ x, const float *__restrict__ d, int n, double scale,
-                                  double *__restrict__ norms)
+__global__ void add_scaled_kernel(double *__restrict__ x, const float *__restrict__ d, int n, double scale)
 {
-    __shared__ double pd[256], px[256];
-    double md = 0.0, mx = 0.0;
     for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256)
-    {
-        const double di = (double)d[i] * scale;
-        const double xi = x[i] + di;
-        x[i] = xi;
-        // a NaN correction counts as infinite, so it can never pass for convergence
-        md = fabs(di) <= DBL_MAX ? fmax(md, fabs(di)) : INFINITY;
-        mx = fmax(mx, fabs(xi));
-    }
-    pd[threadIdx.x] = md;
-    px[threadIdx.x] = mx;
-    __syncthreads();
-    for (int w = 128; w > 0; w >>= 1)
-    {
-        if ((int)threadIdx.x < w)
-        {
-            pd[threadIdx.x] = fmax(pd[threadIdx.x], pd[threadIdx.x + w]);
-            px[threadIdx.x] = fmax(px[threadIdx.x], px[threadIdx.x + w]);
-        }
-        __syncthreads();
-    }
-    if (threadIdx.x == 0)
-    {
-        // max-combine across blocks (non-negative doubles order like their bit patterns)
-        atomicMax((unsigned long long *)&norms[0], (unsigned long long)__double_as_longlong(pd[0]));
-        atomicMax((unsigned long long *)&norms[1], (unsigned long long)__double_as_longlong(px[0]));
-    }
+        x[i] += (double)d[i] * scale;
 }
 
 __global__ void maxabs_kernel(const double *__restrict__ r, int n, double *__restrict__ out)
@@ -371,19 +343,22 @@ int ensure_gmres_ws(int n, int m)
 extern "C" {
 
 // x (FP64, n) holds the initial guess and receives the solution of A x = b.  A64 (FP64)
-// forms the residuals, A32 (FP32 copy of A) the inner GMRES(m) cycles, each run to relative
-// residual inner_tol or m steps.  Stops when max|dx| <= tol * max|x| after a correction; at
-// most max_outer corrections (each one restarts GMRES from the new FP64 residual).  stats[0] = corrections,
-// stats[1] = GMRES steps in total, stats[2] = 1 if converged.  Returns PVL_ERR_NOTCONV when
-// not converged (x then holds the last iterate).
+// forms the residuals, A32 (FP32 copy of A) the inner GMRES(m) cycles.  Each outer step
+// computes r = b - A x in FP64 and stops when max|r| <= tol * max|b|; otherwise one GMRES(m)
+// cycle solves A d = r in FP32 (Jacobi right preconditioning) down to the relative residual
+// max(inner_tol, tol max|b| / (2 max|r|)) -- no further than the final target needs -- and
+// x += d.  At most max_outer corrections.  stats[0] = corrections, stats[1] = GMRES steps in
+// total, stats[2] = 1 if converged; res_out[0] = max|b - A x0| / max|b| of the start,
+// res_out[1] = the final one.  Returns PVL_ERR_NOTCONV when not converged (x then holds the
+// last iterate) and PVL_OK for b = 0 (x = 0).
 int pvl_gmres_refine(int n, const double *A64_dev, const float *A32_dev, const double *b_dev, double *x_dev, double tol,
-                     int m, double inner_tol, int max_outer, int *stats, double *final_ratio)
+                     int m, double inner_tol, int max_outer, int *stats, double *res_out)
 {
     if (int st = ensure_init())
         return st;
-    if (n <= 0 || m <= 0 || m > kMaxM)
+    if (n <= 0 || m <= 0 || m > kMaxM || max_outer < 0)
     {
-        set_error("gmres_refine: n=%d, m=%d (1..%d)", n, m, kMaxM);
+        set_error("gmres_refine: n=%d, m=%d (1..%d), max_outer=%d", n, m, kMaxM, max_outer);
         return PVL_ERR_ARG;
     }
     if (int st = ensure_gmres_ws(n, m))
@@ -395,79 +370,79 @@ int pvl_gmres_refine(int n, const double *A64_dev, const float *A32_dev, const d
         set_error("gmres_refine: n=%d too large", n);
         return PVL_ERR_ARG;
     }
+    ProfScope prof_all("gmres_refine");
     diag_inv_kernel<<<(n + 255) / 256, 256, 0, stream()>>>(A32_dev, n, g_gw.dinv);
     PVL_HIP(hipGetLastError());
-    int total_steps = 0, outer = 0;
+    // norms[0] = max|b|, norms[2] = max|r| of the current outer step
+    PVL_HIP(hipMemsetAsync(g_gw.norms, 0, 4 * sizeof(double), stream()));
+    maxabs_kernel<<<64, 256, 0, stream()>>>(b_dev, n, g_gw.norms);
+    PVL_HIP(hipGetLastError());
+    int total_steps = 0, outer = 0, pending_cycle = 0;
     bool converged = false;
-    double ratio = INFINITY;
-    for (; outer < max_outer; ++outer)
+    double bmax = 0.0, rel0 = INFINITY, rel = INFINITY;
+    for (;; ++outer)
     {
         if (int st = pvl_residual(n, A64_dev, x_dev, b_dev, g_gw.r64, 1))
             return st;
-        // scale the residual to O(1) for the FP32 solve
-        PVL_HIP(hipMemsetAsync(g_gw.norms, 0, 4 * sizeof(double), stream()));
+        PVL_HIP(hipMemsetAsync(g_gw.norms + 2, 0, sizeof(double), stream()));
         maxabs_kernel<<<64, 256, 0, stream()>>>(g_gw.r64, n, g_gw.norms + 2);
-        double rmax = 0;
-        PVL_HIP(hipMemcpyAsync(&rmax, g_gw.norms + 2, sizeof(double), hipMemcpyDeviceToHost, stream()));
+        PVL_HIP(hipGetLastError());
+        // one host synchronisation per outer step: residual norm and the last cycle's result
+        double hn[3];
+        int hres[2] = {0, 0}, herr = 0;
+        PVL_HIP(hipMemcpyAsync(hn, g_gw.norms, 3 * sizeof(double), hipMemcpyDeviceToHost, stream()));
+        if (pending_cycle)
+        {
+            PVL_HIP(hipMemcpyAsync(hres, g_gw.result, 2 * sizeof(int), hipMemcpyDeviceToHost, stream()));
+            PVL_HIP(hipMemcpyAsync(&herr, &g_gw.x->err, sizeof(int), hipMemcpyDeviceToHost, stream()));
+        }
         PVL_HIP(hipStreamSynchronize(stream()));
-        if (rmax == 0.0)
+        if (herr)
+        {
+            set_error("GMRES grid barrier timed out");
+            return PVL_ERR_HIP;
+        }
+        total_steps += hres[0];
+        pending_cycle = 0;
+        bmax = hn[0];
+        const double rmax = hn[2];
+        if (bmax == 0.0)
+        {
+            // b = 0: the solution is x = 0
+            PVL_HIP(hipMemsetAsync(x_dev, 0, sizeof(double) * n, stream()));
+            rel0 = rel = 0.0;
+            converged = true;
+            break;
+        }
+        rel = rmax / bmax;
+        if (outer == 0)
+            rel0 = rel;
+        if (rel <= tol)
         {
             converged = true;
-            ratio = 0.0;
             break;
         }
-        if (!(rmax <= DBL_MAX))
-        {
-            ratio = INFINITY; // non-finite start or residual: report, the caller falls back
-            break;
-        }
+        if (!(rmax <= DBL_MAX) || outer >= max_outer)
+            break; // non-finite start or residual, or out of corrections: the caller falls back
         const double scale = 1.0 / rmax;
         r64_to_32_kernel<<<(n + 255) / 256, 256, 0, stream()>>>(g_gw.r64, g_gw.r32, n, scale);
         PVL_HIP(hipGetLastError());
-        // one GMRES(m) cycle on A d = r; each further correction restarts from the new residual
+        PVL_HIP(hipMemsetAsync(g_gw.x, 0, 16, stream()));
+        int nn = n, mm = m, rp = rpg;
+        float tl = (float)std::max(inner_tol, 0.5 * tol / rel);
+        const float *Ap = A32_dev, *dv = g_gw.dinv, *rh = g_gw.r32;
+        float *dd = g_gw.d32, *Vp = g_gw.V;
+        GmresX *Xp = g_gw.x;
+        int *res = g_gw.result;
+        void *args[] = {&nn, (void *)&Ap, (void *)&dv, (void *)&rh, &dd, &Vp, &mm, &tl, &rp, &Xp, &res};
         {
-            PVL_HIP(hipMemsetAsync(g_gw.x, 0, 16, stream()));
-            int nn = n, mm = m, rp = rpg;
-            float tl = (float)inner_tol;
-            const float *Ap = A32_dev, *dv = g_gw.dinv, *rh = g_gw.r32;
-            float *dd = g_gw.d32, *Vp = g_gw.V;
-            GmresX *Xp = g_gw.x;
-            int *res = g_gw.result;
-            void *args[] = {&nn, (void *)&Ap, (void *)&dv, (void *)&rh, &dd, &Vp, &mm, &tl, &rp, &Xp, &res};
-            {
-                ProfScope prof("gmres_cycle");
-                PVL_HIP(hipLaunchCooperativeKernel((const void *)gmres_cycle_kernel, dim3(G), dim3(256), args, 0,
-                                                   stream()));
-            }
-            PVL_HIP(hipMemsetAsync(g_gw.norms, 0, 2 * sizeof(double), stream()));
-            add_scaled_kernel<<<64, 256, 0, stream()>>>(x_dev, g_gw.d32, n, 1.0 / scale, g_gw.norms);
-            PVL_HIP(hipGetLastError());
-            int hres[2];
-            double hn[2];
-            int herr = 0;
-            PVL_HIP(hipMemcpyAsync(hres, g_gw.result, 2 * sizeof(int), hipMemcpyDeviceToHost, stream()));
-            PVL_HIP(hipMemcpyAsync(hn, g_gw.norms, 2 * sizeof(double), hipMemcpyDeviceToHost, stream()));
-            PVL_HIP(hipMemcpyAsync(&herr, &g_gw.x->err, sizeof(int), hipMemcpyDeviceToHost, stream()));
-            PVL_HIP(hipStreamSynchronize(stream()));
-            if (herr)
-            {
-                set_error("GMRES grid barrier timed out");
-                return PVL_ERR_HIP;
-            }
-            total_steps += hres[0];
-            ratio = hn[1] > 0 ? hn[0] / hn[1] : hn[0];
+            ProfScope prof("gmres_cycle");
+            PVL_HIP(hipLaunchCooperativeKernel((const void *)gmres_cycle_kernel, dim3(G), dim3(256), args, 0,
+                                               stream()));
         }
-        if (ratio <= tol)
-        {
-            converged = true;
-            ++outer;
-            break;
-        }
-        if (!(ratio <= DBL_MAX))
-        {
-            ++outer;
-            break;
-        }
+        add_scaled_kernel<<<64, 256, 0, stream()>>>(x_dev, g_gw.d32, n, 1.0 / scale);
+        PVL_HIP(hipGetLastError());
+        pending_cycle = 1;
     }
     if (stats)
     {
@@ -475,11 +450,14 @@ int pvl_gmres_refine(int n, const double *A64_dev, const float *A32_dev, const d
         stats[1] = total_steps;
         stats[2] = converged ? 1 : 0;
     }
-    if (final_ratio)
-        *final_ratio = ratio;
+    if (res_out)
+    {
+        res_out[0] = rel0;
+        res_out[1] = rel;
+    }
     if (!converged)
     {
-        set_error("GMRES refinement did not reach %.1e (last max|dx|/max|x| = %.3e)", tol, ratio);
+        set_error("GMRES refinement did not reach %.1e (max|r|/max|b| = %.3e after %d corrections)", tol, rel, outer);
         return PVL_ERR_NOTCONV;
     }
     return PVL_OK;

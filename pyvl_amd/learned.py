@@ -2,18 +2,18 @@
 
 A 4-layer MLP (width 256, BF16 products with FP32 accumulation on MFMA, FP32 master weights,
 Adam) is trained on the GPU, by libpvl_hip.so's own forward/backward kernels, on pairs
-that the solver itself generates: per panel, the step's features and the change of its
-circulation since the previous step.  At solve time the previous circulation plus the
-predicted change is the starting point of FP32 GMRES inside FP64 iterative refinement
+that the solver itself generates: per panel, the step's features and how far its
+circulation departs from the linear extrapolation of the two previous steps.  At solve time
+the extrapolation plus the predicted departure is the starting point of FP32 GMRES inside FP64 iterative refinement
 (``pvl_gmres_refine``); a solve that does not converge falls back to the FP64 LU.  There is
 no reference counterpart: the reference factors every step with LAPACK
 (python/pyvl/solver.py:223-227).
 
 Per-panel features (``FEATURES``): the step's right-hand side before and after the wake's
-explicit correction (free stream + body motion, and the wake-induced normal velocity),
-their change since the previous step, the previous step's circulation, the panel's position
-on the wing and its normal, whether it borders the trailing edge, and the body's pitch
-angle, pitch rate and heave velocity.
+explicit correction (free stream + body motion, and the wake-induced normal velocity), its
+first and second differences in time, the previous circulation and its last change, the
+panel's position on the wing and its normal, whether it borders the trailing edge, and the
+body's pitch angle, rate and acceleration and heave rate and acceleration.
 """
 
 from __future__ import annotations
@@ -26,8 +26,9 @@ from . import _lib
 from .device import DeviceArray
 
 FEATURES = (
-    "rhs_body", "rhs_wake", "rhs_total", "d_rhs_total", "gamma_prev", "x_rel", "y_rel", "chord_frac", "n_x", "n_z",
-    "trailing_edge", "pitch", "pitch_rate", "heave_rate",
+    "rhs_body", "rhs_wake", "rhs_total", "d_rhs_total", "dd_rhs_total", "gamma_prev", "d_gamma_prev", "x_rel",
+    "y_rel", "chord_frac", "n_x", "n_z", "trailing_edge", "pitch", "pitch_rate", "pitch_acc", "heave_rate",
+    "heave_acc",
 )
 
 
@@ -108,20 +109,27 @@ class MLP:
 
 
 def gmres_refine(a64: DeviceArray, a32: DeviceArray, b: DeviceArray, x: DeviceArray, tol: float = 1e-12,
-                 m: int = 32, inner_tol: float = 1e-6, max_outer: int = 8):
-    """x <- solution of A x = b (x holds the start).  Returns (converged, corrections,
-    gmres_steps, final max|dx|/max|x|)."""
+                 m: int = 32, inner_tol: float = 1e-7, max_outer: int = 6):
+    """x <- solution of A x = b (x holds the start) by FP32 GMRES in FP64 refinement, to
+    max|b - A x| <= tol max|b|.  Returns (converged, corrections, gmres_steps, start
+    relative residual, final relative residual)."""
     stats = (C.c_int * 3)()
-    ratio = C.c_double()
+    res = (C.c_double * 2)()
     st = _lib.load().pvl_gmres_refine(C.c_int(a64.shape[0]), a64.cptr, a32.cptr, b.cptr, x.cptr, C.c_double(tol),
-                                      C.c_int(m), C.c_double(inner_tol), C.c_int(max_outer), stats, C.byref(ratio))
+                                      C.c_int(m), C.c_double(inner_tol), C.c_int(max_outer), stats, res)
     if st not in (_lib.PVL_OK, _lib.PVL_ERR_NOTCONV):
         _lib.check(st, "pvl_gmres_refine")
-    return st == _lib.PVL_OK, stats[0], stats[1], ratio.value
+    return st == _lib.PVL_OK, stats[0], stats[1], res[0], res[1]
 
 
 class FeatureBuilder:
-    """Per-panel features of the current step (host side; N x len(FEATURES) float32)."""
+    """Per-panel features of the current step (host side; N x len(FEATURES) float32) and the
+    extrapolated circulation the network corrects.
+
+    ``base()``: 2 G(t-1) - G(t-2) once two previous circulations exist, G(t-1) after one
+    step, None before.  The network predicts G(t) - base(); the second difference of the
+    right-hand side (``dd_rhs_total``) and the body's accelerations drive that remainder.
+    """
 
     def __init__(self, solver, pos0: np.ndarray, te_surfaces: np.ndarray, chord_root: float, span: float):
         self.solver = solver
@@ -139,24 +147,38 @@ class FeatureBuilder:
         self.chord_frac = le
         self.te = np.zeros(cps0.shape[0])
         self.te[te_surfaces] = 1.0
-        self.prev_total = None
-        self.prev_gamma = None
+        self.totals: list[np.ndarray] = []   # the last two steps' rhs, newest last
+        self.gammas: list[np.ndarray] = []   # the last two steps' circulations, newest last
+
+    @property
+    def prev_gamma(self):
+        return self.gammas[-1] if self.gammas else None
+
+    def base(self):
+        if len(self.gammas) >= 2:
+            return 2.0 * self.gammas[-1] - self.gammas[-2]
+        return self.prev_gamma
 
     def build(self, rhs_body: np.ndarray, rhs_total: np.ndarray, normals: np.ndarray, kin: tuple) -> np.ndarray:
         n = rhs_body.size
-        d_tot = rhs_total - (self.prev_total if self.prev_total is not None else rhs_total)
-        gp = self.prev_gamma if self.prev_gamma is not None else np.zeros(n)
-        pitch, pitch_rate, heave_rate = kin
+        zero = np.zeros(n)
+        t1 = self.totals[-1] if self.totals else rhs_total
+        t2 = self.totals[-2] if len(self.totals) >= 2 else t1
+        g1 = self.gammas[-1] if self.gammas else zero
+        g2 = self.gammas[-2] if len(self.gammas) >= 2 else g1
+        pitch, pitch_rate, pitch_acc, heave_rate, heave_acc = kin
         f = np.stack([
-            rhs_body, rhs_total - rhs_body, rhs_total, d_tot, gp, self.x_rel, self.y_rel, self.chord_frac,
-            normals[:, 0], normals[:, 2], self.te, np.full(n, np.sin(pitch)), np.full(n, pitch_rate),
-            np.full(n, heave_rate),
+            rhs_body, rhs_total - rhs_body, rhs_total, rhs_total - t1, rhs_total - 2.0 * t1 + t2, g1, g1 - g2,
+            self.x_rel, self.y_rel, self.chord_frac, normals[:, 0], normals[:, 2], self.te,
+            np.full(n, np.sin(pitch)), np.full(n, pitch_rate), np.full(n, pitch_acc), np.full(n, heave_rate),
+            np.full(n, heave_acc),
         ], axis=1)
         return f.astype(np.float32)
 
-    def commit(self, rhs_total: np.ndarray, gamma: np.ndarray) -> None:
-        self.prev_total = rhs_total.copy()
-        self.prev_gamma = gamma.copy()
+    def commit(self, rhs_total, gamma: np.ndarray) -> None:
+        if rhs_total is not None:
+            self.totals = (self.totals + [rhs_total.copy()])[-2:]
+        self.gammas = (self.gammas + [gamma.copy()])[-2:]
 
 
 class Normaliser:
@@ -182,8 +204,10 @@ class Normaliser:
 # ---------------------------------------------------------------------------------------
 # The pitch + heave tapered-wing workload (2,048 panels, free wake): data, training, use.
 def _kinematics(frame, t):
-    th, thd, _, hd = frame._angles(t)
-    return float(th), float(thd), float(hd)
+    """Pitch angle, rate and acceleration; heave rate and acceleration."""
+    th, thd, h, hd = frame._angles(t)
+    w2 = frame.omega * frame.omega
+    return float(th), float(thd), float(-w2 * th), float(hd), float(-w2 * h)
 
 
 def run_c5(kin_seed: int, steps: int, solve: str = "lu64", warm: str = "none", mlp: MLP | None = None,
@@ -191,10 +215,11 @@ def run_c5(kin_seed: int, steps: int, solve: str = "lu64", warm: str = "none", m
     """Run the workload; returns per-step circulations, times and GMRES statistics.
 
     solve "lu64": FP64 LU every step; "gmres": FP32 GMRES in FP64 refinement (LU fallback)
-    started from zero ("none"), the previous step's circulation ("prev") or the previous
-    circulation plus the network's predicted change ("mlp"; zero start at step 0, which has
-    no previous circulation).  ``pre_solve(solver, it)`` is called after the step's system
-    is assembled.  ``solve_s``: features + prediction + solve; ``step_s``: the whole step.
+    started from zero ("none"), the previous step's circulation ("prev"), its linear
+    extrapolation 2 G(t-1) - G(t-2) ("extrap") or that extrapolation plus the network's
+    predicted remainder ("mlp"); steps without enough history start as "prev" does.
+    ``pre_solve(solver, it)`` is called after the step's system is assembled.  ``solve_s``:
+    features + prediction + solve; ``step_s``: the whole step.
     """
     import time
 
@@ -227,11 +252,13 @@ def run_c5(kin_seed: int, steps: int, solve: str = "lu64", warm: str = "none", m
         if need_feats:
             rb, rt, nr = sol.d_b0.numpy(), sol.d_b.numpy(), sol.d_nrm.numpy()
             f = fb.build(rb, rt, nr, _kinematics(sol.frame, t))
-        if warm == "mlp" and fb.prev_gamma is not None:
+        if warm == "mlp" and len(fb.gammas) >= 2:
             dX.copy_from(norm.x(f))
             mlp.forward(dX, dY)
-            x0["v"] = fb.prev_gamma + norm.y_inv(dY.numpy())
-        elif warm in ("prev", "mlp"):
+            x0["v"] = fb.base() + norm.y_inv(dY.numpy())
+        elif warm in ("extrap", "mlp"):
+            x0["v"] = fb.base()
+        elif warm == "prev":
             x0["v"] = fb.prev_gamma
         else:
             x0["v"] = None
@@ -242,10 +269,7 @@ def run_c5(kin_seed: int, steps: int, solve: str = "lu64", warm: str = "none", m
         gammas.append(g)
         if record:
             feats.append(f)
-        if need_feats:
-            fb.commit(rt, g)
-        else:
-            fb.prev_gamma = g
+        fb.commit(rt if need_feats else None, g)
         sol.wake.update(t, sol.mesh, sol.d_pos, sol.d_x, sol.flow)
         sol.iteration += 1
         _lib.call("pvl_synchronize")
@@ -259,8 +283,8 @@ def run_c5(kin_seed: int, steps: int, solve: str = "lu64", warm: str = "none", m
 def generate_training_data(n_steps: int = 2000, seeds=tuple(range(100, 108)), rows: int | None = None):
     """Solver-generated training pairs from FP64-LU runs of the workload, with kinematics
     seeds disjoint from the evaluation seed: per panel and step, the features and the
-    change of circulation since the previous step (step 0, without a previous circulation,
-    is left out)."""
+    circulation's departure from the linear extrapolation 2 G(t-1) - G(t-2) (steps 0 and 1,
+    without two previous circulations, are left out)."""
     per = int(np.ceil(n_steps / len(seeds)))
     X, y = [], []
     left = n_steps
@@ -270,8 +294,8 @@ def generate_training_data(n_steps: int = 2000, seeds=tuple(range(100, 108)), ro
             break
         r = run_c5(s, k, solve="lu64", record=True, rows=rows)
         g = r["gamma"]
-        X.append(r["features"][1:].reshape(-1, len(FEATURES)))
-        y.append((g[1:] - g[:-1]).reshape(-1))
+        X.append(r["features"][2:].reshape(-1, len(FEATURES)))
+        y.append((g[2:] - 2.0 * g[1:-1] + g[:-2]).reshape(-1))
         left -= k
     return np.concatenate(X), np.concatenate(y)
 

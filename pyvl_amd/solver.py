@@ -170,7 +170,9 @@ class UnsteadySolver:
         self.iteration = 0
         self.ir_iters: list[int] = []
         # "gmres": FP32 GMRES in FP64 refinement from warm_start(self) (numpy (n,) or None
-        # for zero), LU64 fallback when it does not converge
+        # for zero) to max|b - A x| <= gmres_tol max|b|, LU64 fallback when it does not
+        # converge; gmres_stats: (corrections, GMRES steps, start and final relative
+        # residual, converged) per solve
         self.warm_start = None
         self.gmres_tol = 1e-12
         self.gmres_stats: list[tuple] = []
@@ -216,8 +218,8 @@ class UnsteadySolver:
                 _lib.call("pvl_memcpy_d2d", self.d_x.cptr, x0.cptr, C.c_size_t(8 * n))
             else:
                 self.d_x.copy_from(np.ascontiguousarray(x0, np.float64))
-            ok, outer, steps, ratio = gmres_refine(self.d_A, self.d_A32, self.d_b, self.d_x, tol=self.gmres_tol)
-            self.gmres_stats.append((outer, steps, ratio, ok))
+            ok, outer, steps, rel0, rel1 = gmres_refine(self.d_A, self.d_A32, self.d_b, self.d_x, tol=self.gmres_tol)
+            self.gmres_stats.append((outer, steps, rel0, rel1, ok))
             if not ok:
                 self.fallbacks += 1
                 self._lu64()
