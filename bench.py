@@ -339,8 +339,9 @@ def run_c5(train_steps: int = 2000, eval_steps: int = 300, seed: int = 7) -> dic
     The network is trained on ``train_steps`` FP64-LU steps from kinematics seeds 100-107
     (disjoint from the evaluation seed), then the evaluation run is repeated with the plain
     FP64 LU and with FP32 GMRES in FP64 refinement started from zero, from the previous
-    step's circulation, from its linear extrapolation and from the extrapolation corrected by
-    the network.  Errors: per step, max|dG| /
+    step's circulation, from its linear extrapolation, from the extrapolation corrected by the
+    network, and from the minimal-residual combination of the last three circulations without
+    and with the network's prediction as a further direction.  Errors: per step, max|dG| /
     max|G| against the FP64 LU run."""
     from pyvl_amd import learned as L
 
@@ -354,7 +355,7 @@ def run_c5(train_steps: int = 2000, eval_steps: int = 300, seed: int = 7) -> dic
     t_train = time.perf_counter() - t0
     log(f"C5 data: {X.shape[0]} samples in {t_gen:.1f} s; training {t_train:.1f} s, losses {losses}")
     runs = {"lu64": L.run_c5(seed, eval_steps, solve="lu64")}
-    for w in ("none", "prev", "extrap", "mlp"):
+    for w in ("none", "prev", "extrap", "mlp", "proj", "proj_mlp"):
         runs["gmres_" + w] = L.run_c5(seed, eval_steps, solve="gmres", warm=w, mlp=net, norm=norm)
     ref = runs["lu64"]["gamma"]
     out = {"panels": int(ref.shape[1]), "train_samples": int(X.shape[0]), "train_data_s": round(t_gen, 2),

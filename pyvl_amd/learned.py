@@ -148,7 +148,11 @@ class FeatureBuilder:
         self.te = np.zeros(cps0.shape[0])
         self.te[te_surfaces] = 1.0
         self.totals: list[np.ndarray] = []   # the last two steps' rhs, newest last
-        self.gammas: list[np.ndarray] = []   # the last two steps' circulations, newest last
+        self.gammas: list[np.ndarray] = []   # the last three steps' circulations, newest last
+
+    @property
+    def history(self) -> list:
+        return self.gammas[::-1]
 
     @property
     def prev_gamma(self):
@@ -178,7 +182,37 @@ class FeatureBuilder:
     def commit(self, rhs_total, gamma: np.ndarray) -> None:
         if rhs_total is not None:
             self.totals = (self.totals + [rhs_total.copy()])[-2:]
-        self.gammas = (self.gammas + [gamma.copy()])[-2:]
+        self.gammas = (self.gammas + [gamma.copy()])[-3:]
+
+
+class Projector:
+    """Minimal-residual start over a few candidate vectors: x0 = V c with
+    c = argmin ||b - A V c||_2 (V = the last solutions and, optionally, the network's
+    prediction).  A V is one FP64 pass over A on the device (``pvl_residual`` with B = 0);
+    the k x k least-squares problem is solved on the host.  The prediction thereby enters
+    as a search direction with its best coefficient, so a poor prediction cannot make the
+    start worse than the previous solutions alone (in the 2-norm of the residual)."""
+
+    def __init__(self, n: int, k_max: int = 4):
+        self.n, self.k_max = n, k_max
+        self.V = DeviceArray((n, k_max))
+        self.Z = DeviceArray.zeros((n, k_max))
+        self.W = DeviceArray((n, k_max))
+
+    def start(self, a: DeviceArray, b_host: np.ndarray, vectors) -> np.ndarray | None:
+        vecs = [v for v in vectors if v is not None]
+        if not vecs:
+            return None
+        k = len(vecs)
+        if k > self.k_max:
+            raise ValueError(f"at most {self.k_max} vectors")
+        V = np.ascontiguousarray(np.stack(vecs, axis=1), np.float64)
+        Vd, Zd, Wd = (d.view(0, (self.n, k)) for d in (self.V, self.Z, self.W))
+        Vd.copy_from(V)
+        _lib.call("pvl_residual", C.c_int(self.n), a.cptr, Vd.cptr, Zd.cptr, Wd.cptr, C.c_int(k))
+        AV = -Wd.numpy()
+        c, *_ = np.linalg.lstsq(AV, b_host, rcond=1e-12)
+        return V @ c
 
 
 class Normaliser:
@@ -216,8 +250,10 @@ def run_c5(kin_seed: int, steps: int, solve: str = "lu64", warm: str = "none", m
 
     solve "lu64": FP64 LU every step; "gmres": FP32 GMRES in FP64 refinement (LU fallback)
     started from zero ("none"), the previous step's circulation ("prev"), its linear
-    extrapolation 2 G(t-1) - G(t-2) ("extrap") or that extrapolation plus the network's
-    predicted remainder ("mlp"); steps without enough history start as "prev" does.
+    extrapolation 2 G(t-1) - G(t-2) ("extrap"), that extrapolation plus the network's
+    predicted remainder ("mlp"), the minimal-residual combination of the last three
+    circulations ("proj", :class:`Projector`) or of those and the network's prediction
+    ("proj_mlp"); steps without enough history start as "prev" does.
     ``pre_solve(solver, it)`` is called after the step's system is assembled.  ``solve_s``:
     features + prediction + solve; ``step_s``: the whole step.
     """
@@ -225,7 +261,7 @@ def run_c5(kin_seed: int, steps: int, solve: str = "lu64", warm: str = "none", m
 
     from .solver import C5, make_c5_solver
 
-    if warm == "mlp" and (mlp is None or norm is None):
+    if warm in ("mlp", "proj_mlp") and (mlp is None or norm is None):
         raise ValueError("warm='mlp' needs a trained network and its normaliser")
     sol = make_c5_solver(kin_seed, solve=solve, rows=rows)
     n = sol.mesh.n_surfaces
@@ -239,7 +275,8 @@ def run_c5(kin_seed: int, steps: int, solve: str = "lu64", warm: str = "none", m
     gammas, times, steps_s, feats = [], [], [], []
     x0 = {"v": None}
     sol.warm_start = lambda s: x0["v"]
-    need_feats = record or warm == "mlp"
+    need_feats = record or warm in ("mlp", "proj_mlp")
+    proj = Projector(n) if warm in ("proj", "proj_mlp") else None
     _lib.call("pvl_synchronize")
     for it in range(steps):
         ts = time.perf_counter()
@@ -252,14 +289,19 @@ def run_c5(kin_seed: int, steps: int, solve: str = "lu64", warm: str = "none", m
         if need_feats:
             rb, rt, nr = sol.d_b0.numpy(), sol.d_b.numpy(), sol.d_nrm.numpy()
             f = fb.build(rb, rt, nr, _kinematics(sol.frame, t))
-        if warm == "mlp" and len(fb.gammas) >= 2:
+        pred = None
+        if warm in ("mlp", "proj_mlp") and len(fb.gammas) >= 2:
             dX.copy_from(norm.x(f))
             mlp.forward(dX, dY)
-            x0["v"] = fb.base() + norm.y_inv(dY.numpy())
-        elif warm in ("extrap", "mlp"):
+            pred = fb.base() + norm.y_inv(dY.numpy())
+        if warm == "mlp":
+            x0["v"] = pred if pred is not None else fb.base()
+        elif warm == "extrap":
             x0["v"] = fb.base()
         elif warm == "prev":
             x0["v"] = fb.prev_gamma
+        elif warm in ("proj", "proj_mlp"):
+            x0["v"] = proj.start(sol.d_A, sol.d_b.numpy(), list(fb.history) + [pred])
         else:
             x0["v"] = None
         sol.solve_system()

@@ -207,7 +207,8 @@ def test_learned_warm_start_gains():
 
     ref = L.run_c5(7, steps, solve="lu64", rows=rows)
     runs = {w: L.run_c5(7, steps, solve="gmres", warm=w, mlp=net, norm=norm, rows=rows,
-                        pre_solve=capture if w == "mlp" else None) for w in ("none", "prev", "extrap", "mlp")}
+                        pre_solve=capture if w == "proj_mlp" else None)
+            for w in ("none", "prev", "extrap", "mlp", "proj", "proj_mlp")}
     for r in runs.values():
         assert r["fallbacks"] == 0
         err = np.max(np.abs(r["gamma"] - ref["gamma"]), axis=1) / np.max(np.abs(ref["gamma"]), axis=1)
@@ -217,9 +218,10 @@ def test_learned_warm_start_gains():
     corr = {w: np.mean([s[0] for s in r["gmres"][2:]]) for w, r in runs.items()}
     res0 = {w: np.median([s[2] for s in r["gmres"][2:]]) for w, r in runs.items()}
     print(f"GMRES steps {steps_}, corrections {corr}, start residual {res0}")
-    assert res0["mlp"] < 0.5 * res0["extrap"] and res0["extrap"] < res0["prev"] < res0["none"]
-    assert steps_["mlp"] < 0.8 * steps_["none"], steps_
-    warm = runs["mlp"]
+    assert res0["extrap"] < res0["prev"] < res0["none"]
+    assert res0["proj_mlp"] < 0.5 * res0["proj"], res0
+    assert steps_["proj_mlp"] < 0.8 * steps_["none"], steps_
+    warm = runs["proj_mlp"]
     # the oracle judges the learned run's circulation at the captured steps
     from pyvl_amd.solver import make_c5_solver
 
