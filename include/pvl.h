@@ -170,9 +170,11 @@ int pvl_lu_factors(const pvl_lu *h, void *lu_dev_out);
 /* R = B - A X in FP64. */
 int pvl_residual(int n, const double *A_dev, const double *X_dev, const double *B_dev, double *R_dev, int nrhs);
 /* Mixed precision solve: h holds the FP32 factorisation of A (pvl_lu_factor on an FP32
- * handle); X_dev receives the FP64 solution of A X = B after FP64 iterative refinement,
- * stopping when every column has max|dx| <= tol * max|x|.  No reference counterpart: the
- * reference only has the FP64 LAPACK solve. */
+ * handle); X_dev receives the FP64 solution of A X = B after FP64 iterative refinement from
+ * X = 0, stopping when every column has max|b - A x| <= tol * max|b| (FP64 residuals).
+ * iters_out: corrections; res_out[nrhs]: final relative residuals.  PVL_ERR_NOTCONV when
+ * the residual stalls or max_iter corrections do not suffice.  No reference counterpart:
+ * the reference only has the FP64 LAPACK solve. */
 int pvl_solve_refine(pvl_lu *h, const double *A_dev, const double *B_dev, double *X_dev, int nrhs, double tol,
                      int max_iter, int *iters_out, double *res_out);
 /* Closed-body deflation A[i][j] += c on the body's diagonal block [r0, r1): the rank-one

@@ -593,47 +593,35 @@ __global__ void f64_to_f32_kernel(const double *__restrict__ a, float *__restric
 }
 
 // Per-column max-norms of D and X (n x nrhs) -> out[0:nrhs] = |D|inf, out[nrhs:2nrhs] = |X|inf.
-__global__ void __launch_bounds__(256) colmax_kernel(const double *__restrict__ D, const double *__restrict__ X,
-                                                     int n, int nrhs, double *__restrict__ out)
+// out[q] = max_i |M[i][q]| (M: n x nrhs row-major); NaN counts as infinite.
+__global__ void __launch_bounds__(256) colabsmax_kernel(const double *__restrict__ M, int n, int nrhs,
+                                                        double *__restrict__ out)
 {
     const int q = blockIdx.x;
-    double md = 0.0, mx = 0.0;
+    double m = 0.0;
     for (int i = threadIdx.x; i < n; i += 256)
     {
-        const double d = fabs(D[(size_t)i * nrhs + q]);
-        md = d <= DBL_MAX ? fmax(md, d) : INFINITY; // a NaN correction never passes as converged
-        mx = fmax(mx, fabs(X[(size_t)i * nrhs + q]));
+        const double v = fabs(M[(size_t)i * nrhs + q]);
+        m = v <= DBL_MAX ? fmax(m, v) : INFINITY;
     }
-    __shared__ double sd[256], sx[256];
-    sd[threadIdx.x] = md;
-    sx[threadIdx.x] = mx;
+    __shared__ double sm[256];
+    sm[threadIdx.x] = m;
     __syncthreads();
     for (int s = 128; s > 0; s >>= 1)
     {
         if (threadIdx.x < s)
-        {
-            sd[threadIdx.x] = fmax(sd[threadIdx.x], sd[threadIdx.x + s]);
-            sx[threadIdx.x] = fmax(sx[threadIdx.x], sx[threadIdx.x + s]);
-        }
+            sm[threadIdx.x] = fmax(sm[threadIdx.x], sm[threadIdx.x + s]);
         __syncthreads();
     }
     if (threadIdx.x == 0)
-    {
-        out[q] = sd[0];
-        out[nrhs + q] = sx[0];
-    }
+        out[q] = sm[0];
 }
 
-// X += (double) D32  and D64 = (double) D32
-__global__ void add_f32_kernel(double *__restrict__ X, const float *__restrict__ D, double *__restrict__ D64,
-                               size_t count)
+// X += (double) D32
+__global__ void add_f32_kernel(double *__restrict__ X, const float *__restrict__ D, size_t count)
 {
     for (size_t i = blockIdx.x * (size_t)256 + threadIdx.x; i < count; i += (size_t)gridDim.x * 256)
-    {
-        const double d = (double)D[i];
-        D64[i] = d;
-        X[i] = X[i] + d;
-    }
+        X[i] = X[i] + (double)D[i];
 }
 
 // ---------------------------------------------------------------------------------------
@@ -974,9 +962,11 @@ int pvl_residual(int n, const double *A_dev, const double *X_dev, const double *
 }
 
 // Mixed precision: FP32 LU of A (handle h, precision FP32, already factored from A_dev),
-// then FP64 iterative refinement x <- x + solve32(b - A x) until every column's
-// max|dx| <= tol * max|x| (or max_iter).  X_dev (n x nrhs FP64) receives the solution.
-// iters_out: refinement steps taken; res_out[nrhs]: final max|dx|/max|x| per column.
+// then FP64 iterative refinement X <- X + solve32(B - A X), starting from X = 0, until every
+// column's FP64 residual satisfies max|r_q| <= tol * max|b_q|.  Stops early, reporting
+// PVL_ERR_NOTCONV, when the worst relative residual fails to halve twice in a row (the
+// FP32 factors are too inaccurate for this matrix) or is not finite.  iters_out:
+// corrections applied; res_out[nrhs]: final max|r_q| / max|b_q| per column.
 int pvl_solve_refine(pvl_lu *h, const double *A_dev, const double *B_dev, double *X_dev, int nrhs, double tol,
                      int max_iter, int *iters_out, double *res_out)
 {
@@ -988,71 +978,72 @@ int pvl_solve_refine(pvl_lu *h, const double *A_dev, const double *B_dev, double
         return PVL_ERR_ARG;
     }
     const int n = h->n;
-    if (nrhs <= 0 || nrhs > kMaxRhs)
+    if (nrhs <= 0 || nrhs > kMaxRhs || max_iter < 1)
     {
-        set_error("solve_refine: nrhs=%d out of range 1..%d", nrhs, kMaxRhs);
+        set_error("solve_refine: nrhs=%d out of range 1..%d (max_iter=%d)", nrhs, kMaxRhs, max_iter);
         return PVL_ERR_ARG;
     }
     const size_t cnt = (size_t)n * nrhs;
     double *R = (double *)scratch(3, sizeof(double) * cnt);
     float *D32 = (float *)scratch(4, sizeof(float) * cnt);
-    double *D64 = (double *)scratch(5, sizeof(double) * cnt);
     double *norms = (double *)scratch(6, sizeof(double) * 2 * nrhs);
-    if (!R || !D32 || !D64 || !norms)
+    if (!R || !D32 || !norms)
         return PVL_ERR_NOMEM;
+    ProfScope prof_all("solve_refine");
     PVL_HIP(hipMemsetAsync(X_dev, 0, sizeof(double) * cnt, stream()));
+    // norms[nrhs + q] = max|b_q| (once); norms[q] = max|r_q| of the current iterate
+    colabsmax_kernel<<<nrhs, 256, 0, stream()>>>(B_dev, n, nrhs, norms + nrhs);
+    PVL_HIP(hipGetLastError());
     std::vector<double> hn(2 * nrhs);
     int it = 0, stalls = 0;
-    double prev_worst = INFINITY;
-    for (; it < max_iter; ++it)
+    double prev_worst = INFINITY, worst = INFINITY;
+    bool converged = false;
+    for (;;)
     {
-        if (int st = pvl_residual(n, A_dev, X_dev, B_dev, R, nrhs))
-            return st;
-        f64_to_f32_kernel<<<512, 256, 0, stream()>>>(R, D32, cnt);
-        PVL_HIP(hipGetLastError());
-        if (int st = solve_impl<float>(h, D32, nrhs))
-            return st;
-        add_f32_kernel<<<512, 256, 0, stream()>>>(X_dev, D32, D64, cnt);
-        PVL_HIP(hipGetLastError());
-        colmax_kernel<<<nrhs, 256, 0, stream()>>>(D64, X_dev, n, nrhs, norms);
+        // X = 0 at the start, so the first residual is B itself
+        const double *Rk = B_dev;
+        if (it > 0)
+        {
+            if (int st = pvl_residual(n, A_dev, X_dev, B_dev, R, nrhs))
+                return st;
+            Rk = R;
+        }
+        colabsmax_kernel<<<nrhs, 256, 0, stream()>>>(Rk, n, nrhs, norms);
         PVL_HIP(hipGetLastError());
         PVL_HIP(hipMemcpyAsync(hn.data(), norms, sizeof(double) * 2 * nrhs, hipMemcpyDeviceToHost, stream()));
         PVL_HIP(hipStreamSynchronize(stream()));
-        bool done = true;
-        double worst = 0.0;
+        worst = 0.0;
         for (int q = 0; q < nrhs; ++q)
         {
             const double r = hn[nrhs + q] > 0 ? hn[q] / hn[nrhs + q] : hn[q];
             if (res_out)
                 res_out[q] = r;
-            if (!(r <= tol))
-                done = false;
             worst = r <= DBL_MAX ? std::max(worst, r) : INFINITY;
         }
-        if (done)
+        if (worst <= tol)
         {
-            ++it;
+            converged = true;
             break;
         }
-        // stalled (the correction no longer shrinks by half twice in a row) or non-finite
         stalls = worst > 0.5 * prev_worst ? stalls + 1 : 0;
         prev_worst = worst;
-        if (stalls >= 2 || !(worst <= DBL_MAX))
-        {
-            ++it;
+        if (it >= max_iter || stalls >= 2 || !(worst <= DBL_MAX))
             break;
-        }
+        f64_to_f32_kernel<<<512, 256, 0, stream()>>>(Rk, D32, cnt);
+        PVL_HIP(hipGetLastError());
+        if (int st = solve_impl<float>(h, D32, nrhs))
+            return st;
+        add_f32_kernel<<<512, 256, 0, stream()>>>(X_dev, D32, cnt);
+        PVL_HIP(hipGetLastError());
+        ++it;
     }
     if (iters_out)
         *iters_out = it;
-    for (int q = 0; q < nrhs; ++q)
+    if (!converged)
     {
-        const double r = hn[nrhs + q] > 0 ? hn[q] / hn[nrhs + q] : hn[q];
-        if (!(r <= tol))
-        {
-            set_error("iterative refinement stalled: column %d at %.3e after %d steps", q, r, it);
-            return PVL_ERR_NOTCONV;
-        }
+        set_error("iterative refinement stopped at max|r|/max|b| = %.3e (target %.1e) after %d corrections", worst,
+                  tol, it);
+        return PVL_ERR_NOTCONV;
     }
     return PVL_OK;
 }

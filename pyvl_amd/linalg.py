@@ -154,8 +154,8 @@ def solve_refine(a: DeviceArray, b: DeviceArray, lu32: LU | None = None, tol: fl
     """Mixed-precision solve of A X = B: FP32 LU (MFMA) + FP64 iterative refinement.
 
     ``a``: (n, n) float64 DeviceArray; ``b``: (n,) or (n, k) float64 DeviceArray.  Refines
-    until every column has max|dx| <= tol * max|x|.  Returns ``(x, iterations, lu32,
-    ratios)``; pass ``lu32`` back to reuse a factorisation.
+    until every column's FP64 residual has max|b - A x| <= tol * max|b|.  Returns ``(x,
+    corrections, lu32, relative residuals)``; pass ``lu32`` back to reuse a factorisation.
     """
     n = a.shape[0]
     if lu32 is None:

@@ -139,7 +139,7 @@ class UnsteadySolver:
     """One body (optionally with a wake), every array of the step resident in HBM.
 
     ``solve``: "lu64" (FP64 LU, the reference's arithmetic) or "ir32" (FP32 LU on MFMA +
-    FP64 iterative refinement to ``ir_tol``).
+    FP64 iterative refinement to the relative residual ``ir_tol``).
     """
 
     def __init__(self, mesh: Mesh, positions0: np.ndarray, frame, flow: FlowConditionsUniform, vortex_limit: float,
