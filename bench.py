@@ -121,7 +121,7 @@ def cpu_free_wake_step(gpu_run) -> dict:
     w = sol.wake
     pos0 = sol.p0
     el_counts, flat = sol.mesh.to_element_connectivity()
-    offs = np.concatenate([[0], np.cumsum(el_counts)])
+    offs = np.concatenate([[0], np.cumsum(el_counts)]).astype(np.int64)
     elements = [flat[offs[i]:offs[i + 1]].tolist() for i in range(len(el_counts))]
     bound = O.mesh_from_elements(pos0.shape[0], elements)
     ow = OracleWake(w.bordering_nodes, w.shedding_lines, w.adjacent_surfaces, C3["tol"], 0.0, C3["rows"])
@@ -178,7 +178,7 @@ def cpu_reference_step(gpu_ref_run, n_steps: int) -> dict:
     w = sol.wake
     pos0 = sol.p0
     el_counts, flat = sol.mesh.to_element_connectivity()
-    offs = np.concatenate([[0], np.cumsum(el_counts)])
+    offs = np.concatenate([[0], np.cumsum(el_counts)]).astype(np.int64)
     elements = [flat[offs[i]:offs[i + 1]].tolist() for i in range(len(el_counts))]
     bound = O.mesh_from_elements(pos0.shape[0], elements)
     ow = OracleWake(w.bordering_nodes, w.shedding_lines, w.adjacent_surfaces, C3["tol"], 0.0, C3["rows"])

@@ -706,13 +706,19 @@ template <typename T> int factor_impl(int n, T *A, int *piv, int *info_out)
                                                stream()));
         }
         // info / err readback is deferred to the end (one sync per factorisation)
-        laswp_kernel<T><<<(n + 255) / 256, 256, 0, stream()>>>(A, n, n, k, jb, piv);
-        PVL_HIP(hipGetLastError());
+        {
+            ProfScope prof(sizeof(T) == 8 ? "lu_laswp_f64" : "lu_laswp_f32");
+            laswp_kernel<T><<<(n + 255) / 256, 256, 0, stream()>>>(A, n, n, k, jb, piv);
+            PVL_HIP(hipGetLastError());
+        }
         const int n2 = n - k - jb;
         if (n2 > 0)
         {
-            trsm_kernel<T><<<(n2 + NB - 1) / NB, 256, 0, stream()>>>(A, n, n, k, jb);
-            PVL_HIP(hipGetLastError());
+            {
+                ProfScope prof(sizeof(T) == 8 ? "lu_trsm_f64" : "lu_trsm_f32");
+                trsm_kernel<T><<<(n2 + NB - 1) / NB, 256, 0, stream()>>>(A, n, n, k, jb);
+                PVL_HIP(hipGetLastError());
+            }
             const dim3 grid((n2 + 63) / 64, (n2 + 63) / 64);
             ProfScope prof(sizeof(T) == 8 ? "lu_gemm_f64" : "lu_gemm_f32");
             gemm_kernel<T><<<grid, 256, 0, stream()>>>(A, n, n2, n2, k, jb);
