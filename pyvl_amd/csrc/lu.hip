@@ -13,9 +13,10 @@
 //   laswp_kernel   applies the panel's interchanges to the other columns.
 //   trsm_kernel    U12 = L11^-1 A12 (one thread per column, L11 in LDS).
 //   gemm_kernel    A22 -= L21 U12 on MFMA (v_mfma_f64_16x16x4f64 / v_mfma_f32_16x16x4f32).
-// Solves use persistent dataflow kernels over 64-row blocks with the inverses of the
-// diagonal blocks precomputed once per factorisation.
+// Solves use persistent dataflow kernels over 64-row blocks; each block's diagonal solve
+// is a wave-level substitution.
 #include "pvl_internal.hpp"
+#include "lu_common.hpp"
 
 #include <cfloat>
 
@@ -111,10 +112,6 @@ __device__ __forceinline__ void grid_barrier(unsigned *counter, unsigned target,
     __syncthreads();
 }
 
-__device__ __forceinline__ bool better(double v1, int p1, double v2, int p2)
-{
-    return v1 > v2 || (v1 == v2 && p1 < p2);
-}
 
 // Exchange area for the panel kernel (double buffered by column parity).
 template <typename T> struct PanelX
@@ -400,59 +397,6 @@ __global__ void __launch_bounds__(256) gemm_kernel(T *__restrict__ A, int lda, i
             }
 }
 
-// Inverses of the 64x64 diagonal blocks: unit lower part -> dinvL, upper part -> dinvU.
-template <typename T>
-__global__ void __launch_bounds__(256) diag_inverse_kernel(const T *__restrict__ A, int lda, int n,
-                                                           T *__restrict__ dinvL, T *__restrict__ dinvU)
-{
-    __shared__ T S[NB][NB + 1];
-    __shared__ T Xl[NB][NB + 1];
-    __shared__ T Xu[NB][NB + 1];
-    const int b = blockIdx.x, r0 = b * NB, nb = min(NB, n - r0);
-    for (int idx = threadIdx.x; idx < NB * NB; idx += 256)
-    {
-        const int r = idx / NB, c = idx % NB;
-        T v;
-        if (r < nb && c < nb)
-            v = A[(size_t)(r0 + r) * lda + r0 + c];
-        else
-            v = (r == c) ? T(1) : T(0); // identity padding
-        S[r][c] = v;
-    }
-    __syncthreads();
-    const int c = threadIdx.x;
-    if (c < NB)
-    {
-        // L X = e_c, L unit lower
-        for (int r = 0; r < NB; ++r)
-        {
-            T s = (r == c) ? T(1) : T(0);
-            for (int t = 0; t < r; ++t)
-                s = fma(-S[r][t], Xl[t][c], s);
-            Xl[r][c] = s;
-        }
-    }
-    else if (c < 2 * NB)
-    {
-        const int cc = c - NB;
-        // U X = e_cc, U upper
-        for (int r = NB - 1; r >= 0; --r)
-        {
-            T s = (r == cc) ? T(1) : T(0);
-            for (int t = r + 1; t < NB; ++t)
-                s = fma(-S[r][t], Xu[t][cc], s);
-            Xu[r][cc] = s / S[r][r];
-        }
-    }
-    __syncthreads();
-    for (int idx = threadIdx.x; idx < NB * NB; idx += 256)
-    {
-        const int r = idx / NB, cc = idx % NB;
-        dinvL[(size_t)b * NB * NB + idx] = Xl[r][cc];
-        dinvU[(size_t)b * NB * NB + idx] = Xu[r][cc];
-    }
-}
-
 // Gather rows: Y[i] = B[perm[i]] (row-major, nrhs columns).
 template <typename T>
 __global__ void __launch_bounds__(256) permute_rows_kernel(const T *__restrict__ B, T *__restrict__ Y, int n, int nrhs,
@@ -468,11 +412,10 @@ __global__ void __launch_bounds__(256) permute_rows_kernel(const T *__restrict__
 // Persistent block triangular solve on B (n x nrhs, row-major), in place.  LOWER: unit
 // lower L, forward order; else upper U, backward order.  Group i owns block row i (in
 // solve order) and folds in every earlier block's solution as soon as its flag shows the
-// current epoch; then multiplies by the precomputed inverse of its diagonal block.
+// current epoch; then solves with its diagonal block.
 template <typename T, bool LOWER>
 __global__ void __launch_bounds__(256) block_trsv_kernel(const T *__restrict__ A, int lda, int n, T *__restrict__ B,
-                                                         int nrhs, const T *__restrict__ dinv,
-                                                         unsigned *__restrict__ flags, unsigned epoch,
+                                                         int nrhs, unsigned *__restrict__ flags, unsigned epoch,
                                                          int *__restrict__ err)
 {
     __shared__ T Ls[NB][NB + 1];
@@ -527,20 +470,41 @@ __global__ void __launch_bounds__(256) block_trsv_kernel(const T *__restrict__ A
         }
         __syncthreads();
     }
-    // x = inv(D_ii) acc
+    // x = D_ii^-1 acc by substitution, as LAPACK's dtrsv orders it (column sweep): wave w
+    // solves the right-hand sides q = w, w + 4, ...; lane = row of the block (identity
+    // padding past n).
     for (int idx = tid; idx < NB * NB; idx += 256)
-        Ls[idx / NB][idx % NB] = dinv[(size_t)ib * NB * NB + idx];
-    __syncthreads();
-    for (int idx = tid; idx < NB * nrhs; idx += 256)
     {
-        const int r = idx / nrhs, q = idx - r * nrhs;
-        if (r < nr)
+        const int r = idx / NB, c = idx % NB;
+        Ls[r][c] = (r < nr && c < nr) ? A[(size_t)(r0 + r) * lda + r0 + c] : (r == c ? T(1) : T(0));
+    }
+    __syncthreads();
+    const int lane = tid & 63, wv = tid >> 6;
+    for (int q = wv; q < nrhs; q += 4)
+    {
+        T x = acc[lane][q];
+        if (LOWER)
         {
-            T s2 = T(0);
-            for (int t = 0; t < NB; ++t)
-                s2 = fma(Ls[r][t], acc[t][q], s2);
-            xstore(&B[(size_t)(r0 + r) * nrhs + q], s2);
+            for (int t = 0; t < NB - 1; ++t)
+            {
+                const T xt = __shfl(x, t);
+                if (lane > t)
+                    x = fma(-Ls[lane][t], xt, x);
+            }
         }
+        else
+        {
+            for (int t = NB - 1; t >= 0; --t)
+            {
+                if (lane == t)
+                    x = x / Ls[t][t];
+                const T xt = __shfl(x, t);
+                if (lane < t)
+                    x = fma(-Ls[lane][t], xt, x);
+            }
+        }
+        if (lane < nr)
+            xstore(&B[(size_t)(r0 + lane) * nrhs + q], x);
     }
     __threadfence();
     __syncthreads();
@@ -680,30 +644,43 @@ template <typename T> int factor_impl(int n, T *A, int *piv, int *info_out)
     PVL_HIP(hipMemsetAsync(X, 0, sizeof(unsigned), stream()));
     PVL_HIP(hipMemsetAsync(status, 0, 2 * sizeof(int), stream()));
     unsigned base = 0;
-    for (int k = 0; k < n; k += NB)
+    // Panel width: 32 up to n = 2048, where single-workgroup register panels cover every
+    // panel and the trailing updates are latency-bound anyway; 64 above.
+    const int nbf = n <= 2048 ? kPanelWgCols : NB;
+    for (int k = 0; k < n; k += nbf)
     {
-        const int jb = std::min(NB, n - k);
+        const int jb = std::min(nbf, n - k);
         const int m = n - k;
-        // rows per group: as many groups as needed for LDS, at most rmax rows each
-        const int G = (m + rmax - 1) / rmax;
-        const int R = (m + G - 1) / G;
-        if (G > kMaxGroups)
+        if (nbf == kPanelWgCols && m <= panel_wg_rows(sizeof(T)))
         {
-            set_error("matrix too tall for the panel kernel (%d rows)", m);
-            return PVL_ERR_ARG;
-        }
-        T *P = A + (size_t)k * n + k;
-        int lda = n, row_base = k;
-        int mm = m, jbb = jb, RR = R;
-        unsigned bb = base;
-        void *args[] = {&P, &lda, &mm, &jbb, &RR, &row_base, &piv, &X, &bb, &status};
-        if (G > 1)
-            base += (unsigned)G * (unsigned)jb;
-        const size_t lds_now = sizeof(T) * (size_t)R * (NB + 1);
-        {
+            T *P = A + (size_t)k * n + k;
             ProfScope prof(sizeof(T) == 8 ? "lu_panel_f64" : "lu_panel_f32");
-            PVL_HIP(hipLaunchCooperativeKernel((const void *)panel_kernel<T>, dim3(G), dim3(256), args, lds_now,
-                                               stream()));
+            if (int st = panel_wg(P, n, m, jb, k, piv, status))
+                return st;
+        }
+        else
+        {
+            // rows per group: as many groups as needed for LDS, at most rmax rows each
+            const int G = (m + rmax - 1) / rmax;
+            const int R = (m + G - 1) / G;
+            if (G > kMaxGroups)
+            {
+                set_error("matrix too tall for the panel kernel (%d rows)", m);
+                return PVL_ERR_ARG;
+            }
+            T *P = A + (size_t)k * n + k;
+            int lda = n, row_base = k;
+            int mm = m, jbb = jb, RR = R;
+            unsigned bb = base;
+            void *args[] = {&P, &lda, &mm, &jbb, &RR, &row_base, &piv, &X, &bb, &status};
+            if (G > 1)
+                base += (unsigned)G * (unsigned)jb;
+            const size_t lds_now = sizeof(T) * (size_t)R * (NB + 1);
+            {
+                ProfScope prof(sizeof(T) == 8 ? "lu_panel_f64" : "lu_panel_f32");
+                PVL_HIP(hipLaunchCooperativeKernel((const void *)panel_kernel<T>, dim3(G), dim3(256), args, lds_now,
+                                                   stream()));
+            }
         }
         // info / err readback is deferred to the end (one sync per factorisation)
         {
@@ -748,8 +725,6 @@ struct pvl_lu
     bool own_lu = false;
     int *piv = nullptr;    // device, n
     int *perm = nullptr;   // device, n: row i of P B is row perm[i] of B
-    void *dinvL = nullptr; // nblk * 64 * 64 (T)
-    void *dinvU = nullptr;
     void *work = nullptr;  // n * kMaxRhs (T) staging for permuted right-hand sides
 };
 
@@ -758,9 +733,7 @@ namespace
 
 template <typename T> int finish_factor(pvl_lu *h)
 {
-    const int n = h->n, nblk = (n + NB - 1) / NB;
-    diag_inverse_kernel<T><<<nblk, 256, 0, stream()>>>((const T *)h->lu, n, n, (T *)h->dinvL, (T *)h->dinvU);
-    PVL_HIP(hipGetLastError());
+    const int n = h->n;
     // permutation from the interchange list (LAPACK order), computed on the host
     std::vector<int> piv(n), perm(n);
     PVL_HIP(hipMemcpyAsync(piv.data(), h->piv, sizeof(int) * n, hipMemcpyDeviceToHost, stream()));
@@ -802,10 +775,9 @@ template <typename T> int solve_impl(pvl_lu *h, T *B, int nrhs)
             const T *Ap = (const T *)h->lu;
             int lda = n, nn = n, nqq = nq;
             T *W = work;
-            const T *dinv = pass == 0 ? (const T *)h->dinvL : (const T *)h->dinvU;
             unsigned *flags = g_ws.flags;
             unsigned ep = epoch;
-            void *args[] = {(void *)&Ap, &lda, &nn, &W, &nqq, (void *)&dinv, &flags, &ep, &err};
+            void *args[] = {(void *)&Ap, &lda, &nn, &W, &nqq, &flags, &ep, &err};
             const void *fn = pass == 0 ? (const void *)block_trsv_kernel<T, true> : (const void *)block_trsv_kernel<T, false>;
             ProfScope prof(sizeof(T) == 8 ? "lu_trsv_f64" : "lu_trsv_f32");
             PVL_HIP(hipLaunchCooperativeKernel(fn, dim3(nblk), dim3(256), args, 0, stream()));
@@ -826,11 +798,8 @@ template <typename T> int solve_impl(pvl_lu *h, T *B, int nrhs)
 int alloc_handle(pvl_lu *h, int n, int prec)
 {
     const size_t es = prec == PVL_FP64 ? 8 : 4;
-    const int nblk = (n + NB - 1) / NB;
     PVL_HIP(hipMalloc(&h->piv, sizeof(int) * n));
     PVL_HIP(hipMalloc(&h->perm, sizeof(int) * n));
-    PVL_HIP(hipMalloc(&h->dinvL, es * (size_t)nblk * NB * NB));
-    PVL_HIP(hipMalloc(&h->dinvU, es * (size_t)nblk * NB * NB));
     PVL_HIP(hipMalloc(&h->work, es * (size_t)n * kMaxRhs));
     return PVL_OK;
 }
@@ -880,8 +849,6 @@ int pvl_lu_destroy(pvl_lu *h)
         (void)hipFree(h->lu);
     (void)hipFree(h->piv);
     (void)hipFree(h->perm);
-    (void)hipFree(h->dinvL);
-    (void)hipFree(h->dinvU);
     (void)hipFree(h->work);
     delete h;
     return PVL_OK;

@@ -54,6 +54,14 @@ namespace pvl
 int point_velocity_impl(const pvl_mesh *m, double tol, const double *pos, const double *circ, unsigned n_pts,
                         const double *pts, double *out, int accumulate, int precision, unsigned n_active,
                         const char *prof_name = "point_velocity");
+
+// Single-workgroup LU panel (lu_panel_f64.hip / lu_panel_f32.hip): factors the m x jb panel
+// at P (row stride lda; matrix rows row_base..) with partial pivoting, piv[row_base + j]
+// receiving the 0-based interchange row, status[0] the first zero pivot (1-based column).
+constexpr int kPanelWgCols = 32;
+constexpr int panel_wg_rows(size_t elem_size) { return elem_size == 8 ? 1024 : 2048; }
+int panel_wg(double *P, int lda, int m, int jb, int row_base, int *piv, int *status);
+int panel_wg(float *P, int lda, int m, int jb, int row_base, int *piv, int *status);
 } // namespace pvl
 
 // Line geometry precomputed once per call: start point, end point, unit direction and
