@@ -154,7 +154,8 @@ def test_closed_body_deflated_solve():
 
 def test_fp64_solve_backward_error_closed_body():
     """Deflated closed-body system (ellipsoid 4:1:0.5, 4,096 panels): the FP64 LU solve is
-    backward stable like LAPACK's (relative residual <= 1e-13) and agrees with it."""
+    backward stable like LAPACK's (relative residual within 10x of LAPACK's own, which is
+    ~4e-13 here) and agrees with it."""
     pos, el = G.ellipsoid(64, 64, axes=(4.0, 1.0, 0.5), sigma_rel=1e-3, seed=0)
     m = Mesh(pos.shape[0], el)
     dp = DeviceArray.from_numpy(pos)
@@ -168,6 +169,7 @@ def test_fp64_solve_backward_error_closed_body():
     ad = a.numpy()
     b = -(nrm.numpy() @ np.array([1.0, 0.05, 0.02]))
     x = LA.LU(n).factor(a).solve(b)
-    assert np.max(np.abs(b - ad @ x)) / np.max(np.abs(b)) < 1e-13
     x_ref = sla.lu_solve(sla.lu_factor(ad), b)
+    rr = lambda v: np.max(np.abs(b - ad @ v)) / np.max(np.abs(b))  # noqa: E731
+    assert rr(x) <= 10.0 * rr(x_ref) + 1e-14, (rr(x), rr(x_ref))
     assert _rel(x, x_ref) < 1e-11
