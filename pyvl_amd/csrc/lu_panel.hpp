@@ -17,13 +17,13 @@ namespace lu_panel
 // registers (thread t owns panel rows t + 512 s, s < RPT; NBP >= jb columns each), for
 // panels short enough to fit (m <= 512 RPT).  The same arithmetic and pivot choice as
 // panel_kernel, but a column costs two workgroup barriers instead of a grid-wide exchange:
-// wave argmax -> 8-entry LDS reduction -> the pivot row's owner publishes it in LDS ->
+// DPP wave argmax -> 8-entry LDS reduction -> the pivot row's owner publishes it in LDS ->
 // swap, scale by the reciprocal pivot, rank-1 update of the owned rows.  The columns are
 // expanded at compile time (PanelStep<J>) so that every register index is static.
 struct PanelShared
 {
-    double red_v[2][8];
-    int red_p[2][8];
+    unsigned long long red_k[2][8];
+    unsigned red_p[2][8];
 };
 
 template <typename T, int NBP, int RPT, int J> struct PanelStep
@@ -36,35 +36,38 @@ template <typename T, int NBP, int RPT, int J> struct PanelStep
         {
             constexpr int par = J & 1;
             const int lane = tid & 63, w = tid >> 6;
-            double bv = -1.0;
-            int bp = INT_MAX;
+            constexpr unsigned NONE = 0xFFFFFFFFu;
+            // this thread's best candidate (rows ascend with s, so a tie keeps the first)
+            unsigned long long key = 0;
+            unsigned row = NONE;
 #pragma unroll
             for (int s = 0; s < RPT; ++s)
             {
                 const int r = tid + NT * s;
-                if (r < m && r >= J)
+                const double v = (double)a[s][J];
+                if (r < m && r >= J && v == v)
                 {
-                    const double v = fabs((double)a[s][J]);
-                    if (better(v, r, bv, bp))
-                        bv = v, bp = r;
+                    const unsigned long long k = pivot_key(v);
+                    if (row == NONE || k > key)
+                        key = k, row = (unsigned)r;
                 }
             }
-            for (int off = 32; off > 0; off >>= 1)
-            {
-                const double ov = __shfl_xor(bv, off);
-                const int op = __shfl_xor(bp, off);
-                if (better(ov, op, bv, bp))
-                    bv = ov, bp = op;
-            }
+            unsigned long long wk;
+            const unsigned wr = wave_argmax(key, row, &wk);
             if (lane == 0)
-                sh.red_v[par][w] = bv, sh.red_p[par][w] = bp;
+                sh.red_k[par][w] = wk, sh.red_p[par][w] = wr;
             __syncthreads();
-            bv = sh.red_v[par][0], bp = sh.red_p[par][0];
+            unsigned long long bk = 0;
+            unsigned br = NONE;
 #pragma unroll
-            for (int q = 1; q < NT / 64; ++q)
-                if (better(sh.red_v[par][q], sh.red_p[par][q], bv, bp))
-                    bv = sh.red_v[par][q], bp = sh.red_p[par][q];
-            const int p = bp == INT_MAX ? J : bp; // every candidate NaN: keep row J in place
+            for (int q = 0; q < NT / 64; ++q)
+            {
+                const unsigned long long k = sh.red_k[par][q];
+                const unsigned r = sh.red_p[par][q];
+                if (k > bk || (k == bk && r < br))
+                    bk = k, br = r;
+            }
+            const int p = br == NONE ? J : (int)br; // every candidate NaN: keep row J in place
             // row J lives in thread J (J < 512), slot 0; the pivot row in thread p % 512, slot p / 512
 #pragma unroll
             for (int s = 0; s < RPT; ++s)
