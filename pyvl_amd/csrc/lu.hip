@@ -142,6 +142,10 @@ __global__ void __launch_bounds__(256) panel_kernel(T *__restrict__ P, int lda, 
     __shared__ double red_v[4];
     __shared__ int red_p[4];
     __shared__ int s_pos, s_grp;
+    // pivots collected in LDS and stored once at the end: a global store before a barrier
+    // makes the barrier wait for it
+    __shared__ int spiv[NB];
+    int first_zero = 0;
 
     const int tid = threadIdx.x, g = blockIdx.x, G = gridDim.x;
     const int r0 = g * R;
@@ -244,8 +248,8 @@ __global__ void __launch_bounds__(256) panel_kernel(T *__restrict__ P, int lda, 
                 for (int c = tid; c < jb; c += 256)
                     L[(p - r0) * LDP + c] = xload(&X->rowj[par][c]);
         }
-        if (g == 0 && tid == 0)
-            piv[row_base + j] = row_base + p;
+        if (tid == 0)
+            spiv[j] = row_base + p; // global store deferred: see the end of the kernel
         __syncthreads();
         // 6. scale by the reciprocal pivot and update the trailing panel columns
         const T pv = prow[j];
@@ -264,12 +268,18 @@ __global__ void __launch_bounds__(256) panel_kernel(T *__restrict__ P, int lda, 
                 }
             }
         }
-        else if (g == 0 && tid == 0)
+        else if (tid == 0 && first_zero == 0)
         {
-            // first exactly-zero pivot, as LAPACK's info (1-based global column)
-            atomicCAS(&status[0], 0, row_base + j + 1);
+            first_zero = row_base + j + 1; // first exactly-zero pivot (1-based global column)
         }
         __syncthreads();
+    }
+    if (g == 0)
+    {
+        for (int j = tid; j < jb; j += 256)
+            piv[row_base + j] = spiv[j];
+        if (tid == 0 && first_zero)
+            atomicCAS(&status[0], 0, first_zero); // LAPACK info
     }
     for (int idx = tid; idx < nr * jb; idx += 256)
     {

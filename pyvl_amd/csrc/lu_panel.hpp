@@ -37,6 +37,10 @@ __global__ void __launch_bounds__(512) panel_wg_kernel(T *__restrict__ P, int ld
     __shared__ T rowj[2][NBP];
     __shared__ unsigned long long red_k[2][NT / 64];
     __shared__ unsigned red_p[2][NT / 64];
+    // pivots stay in LDS until the end: a global store inside the loop would make every
+    // barrier wait for it (the workgroup fence of __syncthreads covers global memory)
+    __shared__ int spiv[NBP];
+    int first_zero = 0; // tid 0: first exactly-zero pivot (1-based column), 0 if none
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     T a[RPT][NBP];
     int done[RPT];
@@ -121,11 +125,11 @@ __global__ void __launch_bounds__(512) panel_wg_kernel(T *__restrict__ P, int ld
         if (tid == J)
             done[0] = J; // row J is now the pivot row: U(J, J..) and L(J, ..J-1), final
         if (tid == 0)
-            piv[row_base + J] = row_base + p;
+            spiv[J] = row_base + p;
         const T pv = prow[par][0];
         const bool zero = !(pv != T(0));
-        if (pv == T(0) && tid == 0)
-            atomicCAS(&status[0], 0, row_base + J + 1); // first exactly-zero pivot (LAPACK info)
+        if (pv == T(0) && tid == 0 && first_zero == 0)
+            first_zero = row_base + J + 1;
         // dgetf2 skips the scaling for a zero pivot (the column below is then zero too) but
         // still applies the (null) rank-1 update; the window shift happens either way
         const T inv = zero ? T(1) : T(1) / pv;
@@ -147,6 +151,11 @@ __global__ void __launch_bounds__(512) panel_wg_kernel(T *__restrict__ P, int ld
             }
         }
     }
+    __syncthreads();
+    if (tid < jb)
+        piv[row_base + tid] = spiv[tid];
+    if (tid == 0 && first_zero)
+        atomicCAS(&status[0], 0, first_zero); // LAPACK info: the first zero pivot
 #pragma unroll
     for (int s = 0; s < RPT; ++s)
     {
