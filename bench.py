@@ -70,7 +70,9 @@ def run_c3(mode: str, prec: int, steps: int, warmup: int, profile: bool):
         f"(last 50 growth steps {np.mean(grow[-50:]) * 1e3:.2f} ms/step)")
     sol.run(warmup)
     if profile:
+        # inside the timed region only the dominant kernel carries HIP events
         lib.pvl_prof_reset()
+        lib.pvl_prof_filter(b"wake_self_velocity")
         lib.pvl_prof_enable(1)
     _lib.call("pvl_synchronize")
     t0 = time.perf_counter()
@@ -78,16 +80,25 @@ def run_c3(mode: str, prec: int, steps: int, warmup: int, profile: bool):
         sol.step()
     _lib.call("pvl_synchronize")
     timed = time.perf_counter() - t0
-    prof = {}
+    prof, breakdown = {}, {}
     if profile:
+        prof["wake_self_velocity"] = prof_get(lib, "wake_self_velocity")
+        # per-kernel breakdown from further (untimed) full-wake steps with every scope timed
+        lib.pvl_prof_filter(None)
+        lib.pvl_prof_reset()
+        n_b = min(steps, 10)
+        for _ in range(n_b):
+            sol.step()
+        _lib.call("pvl_synchronize")
         for name in ("wake_self_velocity", "wake_bound_velocity", "wake_explicit", "induction_matrix3",
-                     "lu_panel_f64", "lu_gemm_f64", "lu_trsv_f64"):
-            prof[name] = prof_get(lib, name)
+                     "lu_panel_f64", "lu_laswp_f64", "lu_trsm_f64", "lu_gemm_f64", "lu_trsv_f64"):
+            ms, cnt, _ = prof_get(lib, name)
+            breakdown[name] = {"ms_per_step": round(ms / n_b, 4), "launches_per_step": cnt / n_b}
         lib.pvl_prof_enable(0)
     log(f"{mode}/{'fp64' if prec == 0 else 'fp32'}: {steps} full-wake steps in {timed:.3f} s "
         f"-> {steps / timed:.1f} steps/s")
     return dict(sol=sol, timed_s=timed, steps_per_s=steps / timed, ms_per_step=1e3 * timed / steps,
-                grow_400_s=grow_s, prof=prof)
+                grow_400_s=grow_s, prof=prof, breakdown=breakdown)
 
 
 def roofline_from(prof) -> dict:
@@ -106,7 +117,7 @@ def roofline_from(prof) -> dict:
         "note": "FP64 vector-ALU bound (MI355X FP64 vector peak = FP64 MFMA peak = 78.6 TFLOP/s); "
                 f"algorithmic {FLOP_PER_PAIR + FLOP_WEIGHT:.0f} FLOP per (segment, point) pair, "
                 f"{pairs / max(cnt, 1):.4g} pairs per launch, avg {avg_ms:.3f} ms over {cnt} launches "
-                "(HIP events on the library stream, timed region only)",
+                "(HIP events on the library stream around this kernel only, over the timed region)",
     }
 
 
@@ -397,8 +408,7 @@ def main() -> int:
     _lib.call("pvl_init", 0)
     head = run_c3("free", 0, args.steps, args.warmup, profile=True)
     extra = {"c3_grow_400_steps_s": round(head["grow_400_s"], 3)}
-    prof = {k: {"ms": round(v[0], 3), "launches": v[1]} for k, v in head["prof"].items()}
-    extra["c3_timed_region_kernels"] = prof
+    extra["c3_kernel_breakdown_ms_per_step"] = head["breakdown"]
     cpu = None
     if not args.headline_only:
         f32 = run_c3("free", 1, args.steps, args.warmup, profile=False)

@@ -69,6 +69,9 @@ int pvl_memset_zero(void *dev_dst, size_t bytes);
 /* Kernel timing: when enabled, the main kernel of each entry point is bracketed by HIP
  * events on the library stream; totals are kept per entry-point name. */
 int pvl_prof_enable(int on);
+/* Restrict the timing to scopes named `name` (NULL: time every scope), so that a timed
+ * region carries the events of one kernel only. */
+int pvl_prof_filter(const char *name);
 int pvl_prof_reset(void);
 int pvl_prof_query(const char *name, double *total_ms, long *count);
 /* Work units accumulated under `name` (for induction kernels: (line, point) pairs). */

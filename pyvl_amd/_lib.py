@@ -42,6 +42,7 @@ SIGNATURES: dict[str, tuple] = {
     "pvl_memcpy_d2d": (C.c_int, [_dp, _dp, C.c_size_t]),
     "pvl_memset_zero": (C.c_int, [_dp, C.c_size_t]),
     "pvl_prof_enable": (C.c_int, [C.c_int]),
+    "pvl_prof_filter": (C.c_int, [C.c_char_p]),
     "pvl_prof_reset": (C.c_int, []),
     "pvl_prof_query": (C.c_int, [C.c_char_p, C.POINTER(C.c_double), C.POINTER(C.c_long)]),
     "pvl_prof_work": (C.c_int, [C.c_char_p, C.POINTER(C.c_double)]),

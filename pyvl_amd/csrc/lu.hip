@@ -757,6 +757,25 @@ template <typename T> int finish_factor(pvl_lu *h)
     return PVL_OK;
 }
 
+// Workgroups of the triangular-solve kernel that the device holds at once (both passes
+// use the same resources); 0 if the query fails, which selects the cooperative launch.
+template <typename T> int trsv_resident_groups()
+{
+    static int cached = -1;
+    if (cached < 0)
+    {
+        int per_cu = 0, dev = 0, cus = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+            hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)block_trsv_kernel<T, true>, 256, 0) !=
+                hipSuccess)
+            cached = 0;
+        else
+            cached = per_cu * cus;
+    }
+    return cached;
+}
+
 template <typename T> int solve_impl(pvl_lu *h, T *B, int nrhs)
 {
     const int n = h->n, nblk = (n + NB - 1) / NB;
@@ -790,7 +809,13 @@ template <typename T> int solve_impl(pvl_lu *h, T *B, int nrhs)
             void *args[] = {(void *)&Ap, &lda, &nn, &W, &nqq, &flags, &ep, &err};
             const void *fn = pass == 0 ? (const void *)block_trsv_kernel<T, true> : (const void *)block_trsv_kernel<T, false>;
             ProfScope prof(sizeof(T) == 8 ? "lu_trsv_f64" : "lu_trsv_f32");
-            PVL_HIP(hipLaunchCooperativeKernel(fn, dim3(nblk), dim3(256), args, 0, stream()));
+            // A group only waits for groups of lower index (earlier blocks in solve order),
+            // which in-order dispatch has already placed; when the whole grid is resident at
+            // once anyway a plain launch suffices and avoids the cooperative-launch path.
+            if (nblk <= trsv_resident_groups<T>())
+                PVL_HIP(hipLaunchKernel(fn, dim3(nblk), dim3(256), args, 0, stream()));
+            else
+                PVL_HIP(hipLaunchCooperativeKernel(fn, dim3(nblk), dim3(256), args, 0, stream()));
         }
         PVL_HIP(hipMemcpyAsync(B, work, sizeof(T) * (size_t)n * nq, hipMemcpyDeviceToDevice, stream()));
         int e = 0;

@@ -121,6 +121,7 @@ struct Pending
     double work;
 };
 static bool g_prof = false;
+static std::string g_prof_only; // empty: every scope
 static std::map<std::string, ProfEntry> g_prof_tab;
 static std::vector<Pending> g_pending;
 static std::vector<hipEvent_t> g_event_pool;
@@ -159,7 +160,7 @@ static void resolve_pending()
 
 ProfScope::ProfScope(const char *n, double work_units) : name(n), work(work_units)
 {
-    if (!g_prof)
+    if (!g_prof || (!g_prof_only.empty() && g_prof_only != n))
         return;
     ev0 = take_event();
     ev1 = take_event();
@@ -289,6 +290,12 @@ int pvl_memset_zero(void *dst, size_t bytes)
     if (bytes == 0)
         return PVL_OK;
     PVL_HIP(hipMemsetAsync(dst, 0, bytes, g_stream));
+    return PVL_OK;
+}
+
+int pvl_prof_filter(const char *name)
+{
+    g_prof_only = name ? name : "";
     return PVL_OK;
 }
 
