@@ -388,6 +388,11 @@ def run_c5(train_steps: int = 2000, eval_steps: int = 300, seed: int = 7) -> dic
             "max_rel_err_vs_fp64_lu": float(np.max(err)),
         }
         log(f"C5 {k}: {out[k]}")
+    r0 = {k: out[k]["start_rel_residual_median"] for k in runs if k != "lu64"}
+    out["start_residual_gain_network_over_extrapolation"] = round(r0["gmres_extrap"] / r0["gmres_mlp"], 3)
+    out["start_residual_gain_network_over_projection"] = round(r0["gmres_proj"] / r0["gmres_proj_mlp"], 3)
+    out["run_speedup_gmres_proj_over_fp64_lu"] = round(out["lu64"]["run_s"] / out["gmres_proj"]["run_s"], 3)
+    out["run_speedup_gmres_mlp_over_fp64_lu"] = round(out["lu64"]["run_s"] / out["gmres_mlp"]["run_s"], 3)
     return out
 
 
