@@ -322,11 +322,14 @@ def run_c5(kin_seed: int, steps: int, solve: str = "lu64", warm: str = "none", m
     }
 
 
-def generate_training_data(n_steps: int = 2000, seeds=tuple(range(100, 108)), rows: int | None = None):
-    """Solver-generated training pairs from FP64-LU runs of the workload, with kinematics
-    seeds disjoint from the evaluation seed: per panel and step, the features and the
+def generate_training_data(n_steps: int = 2000, seeds=tuple(range(100, 108)), rows: int | None = None,
+                           solve: str = "gmres"):
+    """Solver-generated training pairs from runs of the workload with kinematics seeds
+    disjoint from the evaluation seed: per panel and step, the features and the
     circulation's departure from the linear extrapolation 2 G(t-1) - G(t-2) (steps 0 and 1,
-    without two previous circulations, are left out)."""
+    without two previous circulations, are left out).  ``solve`` "gmres": the refined GMRES
+    started by the minimal-residual projection (the same circulations as the FP64 LU to
+    ~1e-12, at a fraction of the cost); "lu64": the FP64 LU."""
     per = int(np.ceil(n_steps / len(seeds)))
     X, y = [], []
     left = n_steps
@@ -334,7 +337,7 @@ def generate_training_data(n_steps: int = 2000, seeds=tuple(range(100, 108)), ro
         k = min(per, left)
         if k <= 0:
             break
-        r = run_c5(s, k, solve="lu64", record=True, rows=rows)
+        r = run_c5(s, k, solve=solve, warm="proj" if solve == "gmres" else "none", record=True, rows=rows)
         g = r["gamma"]
         X.append(r["features"][2:].reshape(-1, len(FEATURES)))
         y.append((g[2:] - 2.0 * g[1:-1] + g[:-2]).reshape(-1))
