@@ -13,8 +13,8 @@
 //   laswp_kernel   applies the panel's interchanges to the other columns.
 //   trsm_kernel    U12 = L11^-1 A12 (one thread per column, L11 in LDS).
 //   gemm_kernel    A22 -= L21 U12 on MFMA (v_mfma_f64_16x16x4f64 / v_mfma_f32_16x16x4f32).
-// Solves use persistent dataflow kernels over 64-row blocks; each block's diagonal solve
-// is a wave-level substitution.
+// Solves run one launch per 64-row block (update of the later blocks, then a wave-level
+// substitution with the next diagonal block).
 #include "pvl_internal.hpp"
 #include "lu_common.hpp"
 
@@ -419,21 +419,24 @@ __global__ void __launch_bounds__(256) permute_rows_kernel(const T *__restrict__
     Y[idx] = B[(size_t)perm[i] * nrhs + q];
 }
 
-// Persistent block triangular solve on B (n x nrhs, row-major), in place.  LOWER: unit
-// lower L, forward order; else upper U, backward order.  Group i owns block row i (in
-// solve order) and folds in every earlier block's solution as soon as its flag shows the
-// current epoch; then solves with its diagonal block.
+// Block triangular solve on B (n x nrhs, row-major), in place, one launch per 64-row block
+// in solve order.  LOWER: unit lower L, forward order; else upper U, backward order.
+// Launch `step` (>= 0) subtracts the contribution of the block solved by the previous
+// launch (solve-order index `step`) from every later block, and workgroup 0 then solves the
+// next block (solve-order index step + 1) with its diagonal block by substitution, as
+// LAPACK's dtrsv orders it (column sweep; wave w takes the right-hand sides w, w + 4, ...,
+// lane = row).  step = -1 only solves block 0.  No workgroup waits for another: the launch
+// boundary orders the blocks.
 template <typename T, bool LOWER>
-__global__ void __launch_bounds__(256) block_trsv_kernel(const T *__restrict__ A, int lda, int n, T *__restrict__ B,
-                                                         int nrhs, unsigned *__restrict__ flags, unsigned epoch,
-                                                         int *__restrict__ err)
+__global__ void __launch_bounds__(256) trsv_step_kernel(const T *__restrict__ A, int lda, int n, T *__restrict__ B,
+                                                        int nrhs, int step)
 {
     __shared__ T Ls[NB][NB + 1];
     __shared__ T xs[NB][kMaxRhs + 1];
     __shared__ T acc[NB][kMaxRhs + 1];
     const int nblk = (n + NB - 1) / NB;
-    const int step = blockIdx.x;
-    const int ib = LOWER ? step : nblk - 1 - step;
+    const int t = step + 1 + blockIdx.x; // target block, solve order
+    const int ib = LOWER ? t : nblk - 1 - t;
     const int r0 = ib * NB, nr = min(NB, n - r0);
     const int tid = threadIdx.x;
     for (int idx = tid; idx < NB * nrhs; idx += 256)
@@ -441,24 +444,10 @@ __global__ void __launch_bounds__(256) block_trsv_kernel(const T *__restrict__ A
         const int r = idx / nrhs, q = idx - r * nrhs;
         acc[r][q] = (r < nr) ? B[(size_t)(r0 + r) * nrhs + q] : T(0);
     }
-    for (int s = 0; s < step; ++s)
+    if (step >= 0)
     {
-        const int jb = LOWER ? s : nblk - 1 - s;
-        const int c0 = jb * NB, nc = min(NB, n - c0);
-        if (tid == 0)
-        {
-            unsigned spins = 0;
-            while (__hip_atomic_load(&flags[jb], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) != epoch)
-            {
-                __builtin_amdgcn_s_sleep(1);
-                if (++spins > kSpinLimit || __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
-                {
-                    __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                    break;
-                }
-            }
-        }
-        __syncthreads();
+        const int sb = LOWER ? step : nblk - 1 - step;
+        const int c0 = sb * NB, nc = min(NB, n - c0);
         for (int idx = tid; idx < NB * NB; idx += 256)
         {
             const int r = idx / NB, c = idx % NB;
@@ -466,23 +455,30 @@ __global__ void __launch_bounds__(256) block_trsv_kernel(const T *__restrict__ A
         }
         for (int idx = tid; idx < NB * nrhs; idx += 256)
         {
-            const int t = idx / nrhs, q = idx - t * nrhs;
-            xs[t][q] = (t < nc) ? xload(&B[(size_t)(c0 + t) * nrhs + q]) : T(0);
+            const int r = idx / nrhs, q = idx - r * nrhs;
+            xs[r][q] = (r < nc) ? B[(size_t)(c0 + r) * nrhs + q] : T(0);
         }
         __syncthreads();
         for (int idx = tid; idx < NB * nrhs; idx += 256)
         {
             const int r = idx / nrhs, q = idx - r * nrhs;
             T s2 = acc[r][q];
-            for (int t = 0; t < NB; ++t)
-                s2 = fma(-Ls[r][t], xs[t][q], s2);
+            for (int c = 0; c < NB; ++c)
+                s2 = fma(-Ls[r][c], xs[c][q], s2);
             acc[r][q] = s2;
         }
         __syncthreads();
     }
-    // x = D_ii^-1 acc by substitution, as LAPACK's dtrsv orders it (column sweep): wave w
-    // solves the right-hand sides q = w, w + 4, ...; lane = row of the block (identity
-    // padding past n).
+    if (blockIdx.x != 0)
+    {
+        for (int idx = tid; idx < nr * nrhs; idx += 256)
+        {
+            const int r = idx / nrhs, q = idx - r * nrhs;
+            B[(size_t)(r0 + r) * nrhs + q] = acc[r][q];
+        }
+        return;
+    }
+    // this block is complete: x = D^-1 acc (identity padding past n)
     for (int idx = tid; idx < NB * NB; idx += 256)
     {
         const int r = idx / NB, c = idx % NB;
@@ -495,31 +491,27 @@ __global__ void __launch_bounds__(256) block_trsv_kernel(const T *__restrict__ A
         T x = acc[lane][q];
         if (LOWER)
         {
-            for (int t = 0; t < NB - 1; ++t)
+            for (int k = 0; k < NB - 1; ++k)
             {
-                const T xt = __shfl(x, t);
-                if (lane > t)
-                    x = fma(-Ls[lane][t], xt, x);
+                const T xk = __shfl(x, k);
+                if (lane > k)
+                    x = fma(-Ls[lane][k], xk, x);
             }
         }
         else
         {
-            for (int t = NB - 1; t >= 0; --t)
+            for (int k = NB - 1; k >= 0; --k)
             {
-                if (lane == t)
-                    x = x / Ls[t][t];
-                const T xt = __shfl(x, t);
-                if (lane < t)
-                    x = fma(-Ls[lane][t], xt, x);
+                if (lane == k)
+                    x = x / Ls[k][k];
+                const T xk = __shfl(x, k);
+                if (lane < k)
+                    x = fma(-Ls[lane][k], xk, x);
             }
         }
         if (lane < nr)
-            xstore(&B[(size_t)(r0 + lane) * nrhs + q], x);
+            B[(size_t)(r0 + lane) * nrhs + q] = x;
     }
-    __threadfence();
-    __syncthreads();
-    if (tid == 0)
-        __hip_atomic_store(&flags[ib], epoch, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // R = B - A X  (FP64; A n x n row-major; X, B, R n x nrhs).  One wave per row.
@@ -603,32 +595,17 @@ __global__ void add_f32_kernel(double *__restrict__ X, const float *__restrict__
 struct Workspace
 {
     void *panelx = nullptr;
-    unsigned *flags = nullptr; // per block, epoch-tagged
-    int *err = nullptr;        // err[0]: first zero pivot (factor), err[1]: a bounded wait timed out
-    unsigned epoch = 0;
-    int nblk_cap = 0;
+    int *err = nullptr; // err[0]: first zero pivot (factor), err[1]: a bounded wait timed out
 };
 Workspace g_ws;
 
-int ensure_ws(int nblk)
+int ensure_ws()
 {
     if (!g_ws.panelx)
     {
         PVL_HIP(hipMalloc(&g_ws.panelx, sizeof(PanelX<double>)));
         PVL_HIP(hipMalloc(&g_ws.err, 2 * sizeof(int)));
         PVL_HIP(hipMemset(g_ws.err, 0, 2 * sizeof(int)));
-    }
-    if (nblk > g_ws.nblk_cap)
-    {
-        if (g_ws.flags)
-        {
-            PVL_HIP(hipStreamSynchronize(stream()));
-            PVL_HIP(hipFree(g_ws.flags));
-        }
-        PVL_HIP(hipMalloc(&g_ws.flags, sizeof(unsigned) * nblk));
-        PVL_HIP(hipMemset(g_ws.flags, 0, sizeof(unsigned) * nblk));
-        g_ws.nblk_cap = nblk;
-        g_ws.epoch = 0;
     }
     return PVL_OK;
 }
@@ -637,8 +614,7 @@ template <typename T> int panel_rows_max() { return sizeof(T) == 8 ? 256 : 512; 
 
 template <typename T> int factor_impl(int n, T *A, int *piv, int *info_out)
 {
-    const int nblk = (n + NB - 1) / NB;
-    if (int st = ensure_ws(nblk))
+    if (int st = ensure_ws())
         return st;
     static bool attr_set[2] = {false, false};
     const int rmax = panel_rows_max<T>();
@@ -757,29 +733,10 @@ template <typename T> int finish_factor(pvl_lu *h)
     return PVL_OK;
 }
 
-// Workgroups of the triangular-solve kernel that the device holds at once (both passes
-// use the same resources); 0 if the query fails, which selects the cooperative launch.
-template <typename T> int trsv_resident_groups()
-{
-    static int cached = -1;
-    if (cached < 0)
-    {
-        int per_cu = 0, dev = 0, cus = 0;
-        if (hipGetDevice(&dev) != hipSuccess ||
-            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
-            hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)block_trsv_kernel<T, true>, 256, 0) !=
-                hipSuccess)
-            cached = 0;
-        else
-            cached = per_cu * cus;
-    }
-    return cached;
-}
-
 template <typename T> int solve_impl(pvl_lu *h, T *B, int nrhs)
 {
     const int n = h->n, nblk = (n + NB - 1) / NB;
-    if (int st = ensure_ws(nblk))
+    if (int st = ensure_ws())
         return st;
     T *work = (T *)h->work;
     for (int q0 = 0; q0 < nrhs; q0 += kMaxRhs)
@@ -796,36 +753,17 @@ template <typename T> int solve_impl(pvl_lu *h, T *B, int nrhs)
             return PVL_ERR_ARG;
         }
         PVL_HIP(hipGetLastError());
-        int *err = g_ws.err + 1;
-        PVL_HIP(hipMemsetAsync(err, 0, sizeof(int), stream()));
-        for (int pass = 0; pass < 2; ++pass)
         {
-            const unsigned epoch = ++g_ws.epoch;
-            const T *Ap = (const T *)h->lu;
-            int lda = n, nn = n, nqq = nq;
-            T *W = work;
-            unsigned *flags = g_ws.flags;
-            unsigned ep = epoch;
-            void *args[] = {(void *)&Ap, &lda, &nn, &W, &nqq, &flags, &ep, &err};
-            const void *fn = pass == 0 ? (const void *)block_trsv_kernel<T, true> : (const void *)block_trsv_kernel<T, false>;
             ProfScope prof(sizeof(T) == 8 ? "lu_trsv_f64" : "lu_trsv_f32");
-            // A group only waits for groups of lower index (earlier blocks in solve order),
-            // which in-order dispatch has already placed; when the whole grid is resident at
-            // once anyway a plain launch suffices and avoids the cooperative-launch path.
-            if (nblk <= trsv_resident_groups<T>())
-                PVL_HIP(hipLaunchKernel(fn, dim3(nblk), dim3(256), args, 0, stream()));
-            else
-                PVL_HIP(hipLaunchCooperativeKernel(fn, dim3(nblk), dim3(256), args, 0, stream()));
+            const T *Ap = (const T *)h->lu;
+            for (int st = -1; st < nblk - 1; ++st)
+                trsv_step_kernel<T, true><<<st < 0 ? 1 : nblk - 1 - st, 256, 0, stream()>>>(Ap, n, n, work, nq, st);
+            PVL_HIP(hipGetLastError());
+            for (int st = -1; st < nblk - 1; ++st)
+                trsv_step_kernel<T, false><<<st < 0 ? 1 : nblk - 1 - st, 256, 0, stream()>>>(Ap, n, n, work, nq, st);
+            PVL_HIP(hipGetLastError());
         }
         PVL_HIP(hipMemcpyAsync(B, work, sizeof(T) * (size_t)n * nq, hipMemcpyDeviceToDevice, stream()));
-        int e = 0;
-        PVL_HIP(hipMemcpyAsync(&e, err, sizeof(int), hipMemcpyDeviceToHost, stream()));
-        PVL_HIP(hipStreamSynchronize(stream()));
-        if (e)
-        {
-            set_error("triangular solve wait timed out");
-            return PVL_ERR_HIP;
-        }
     }
     return PVL_OK;
 }
