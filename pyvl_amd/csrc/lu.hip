@@ -419,13 +419,25 @@ __global__ void __launch_bounds__(256) permute_rows_kernel(const T *__restrict__
     Y[idx] = B[(size_t)perm[i] * nrhs + q];
 }
 
+// Value of lane k (wave-uniform k) in every lane, through readlane (no LDS round trip).
+__device__ __forceinline__ double bcast_lane(double v, int k)
+{
+    const unsigned long long u = (unsigned long long)__double_as_longlong(v);
+    const unsigned lo = __builtin_amdgcn_readlane((unsigned)u, k);
+    const unsigned hi = __builtin_amdgcn_readlane((unsigned)(u >> 32), k);
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+__device__ __forceinline__ float bcast_lane(float v, int k)
+{
+    return __uint_as_float(__builtin_amdgcn_readlane(__float_as_uint(v), k));
+}
+
 // Block triangular solve on B (n x nrhs, row-major), in place, one launch per 64-row block
 // in solve order.  LOWER: unit lower L, forward order; else upper U, backward order.
 // Launch `step` (>= 0) subtracts the contribution of the block solved by the previous
 // launch (solve-order index `step`) from every later block, and workgroup 0 then solves the
 // next block (solve-order index step + 1) with its diagonal block by substitution, as
-// LAPACK's dtrsv orders it (column sweep; wave w takes the right-hand sides w, w + 4, ...,
-// lane = row).  step = -1 only solves block 0.  No workgroup waits for another: the launch
+// LAPACK's dtrsv orders it (column sweep; lane = row).  step = -1 only solves block 0.  No workgroup waits for another: the launch
 // boundary orders the blocks.
 template <typename T, bool LOWER>
 __global__ void __launch_bounds__(256) trsv_step_kernel(const T *__restrict__ A, int lda, int n, T *__restrict__ B,
@@ -485,32 +497,40 @@ __global__ void __launch_bounds__(256) trsv_step_kernel(const T *__restrict__ A,
         Ls[r][c] = (r < nr && c < nr) ? A[(size_t)(r0 + r) * lda + r0 + c] : (r == c ? T(1) : T(0));
     }
     __syncthreads();
+    // wave wv holds the right-hand sides wv, wv + 4, ... (at most kMaxRhs / 4) in registers
+    // and advances them together; the current row's value is broadcast with readlane.
     const int lane = tid & 63, wv = tid >> 6;
-    for (int q = wv; q < nrhs; q += 4)
+    constexpr int QW = kMaxRhs / 4;
+    T x[QW];
+#pragma unroll
+    for (int j = 0; j < QW; ++j)
     {
-        T x = acc[lane][q];
-        if (LOWER)
+        const int q = wv + 4 * j;
+        x[j] = q < nrhs ? acc[lane][q] : T(0);
+    }
+    const T lrow_diag = Ls[lane][lane];
+    for (int kk = 0; kk < NB; ++kk)
+    {
+        const int k = LOWER ? kk : NB - 1 - kk;
+        const T lk = Ls[lane][k];
+#pragma unroll
+        for (int j = 0; j < QW; ++j)
         {
-            for (int k = 0; k < NB - 1; ++k)
-            {
-                const T xk = __shfl(x, k);
-                if (lane > k)
-                    x = fma(-Ls[lane][k], xk, x);
-            }
+            if (wv + 4 * j >= nrhs) // uniform
+                break;
+            if (!LOWER && lane == k)
+                x[j] = x[j] / lrow_diag;
+            const T xk = bcast_lane(x[j], k);
+            if (LOWER ? lane > k : lane < k)
+                x[j] = fma(-lk, xk, x[j]);
         }
-        else
-        {
-            for (int k = NB - 1; k >= 0; --k)
-            {
-                if (lane == k)
-                    x = x / Ls[k][k];
-                const T xk = __shfl(x, k);
-                if (lane < k)
-                    x = fma(-Ls[lane][k], xk, x);
-            }
-        }
-        if (lane < nr)
-            B[(size_t)(r0 + lane) * nrhs + q] = x;
+    }
+#pragma unroll
+    for (int j = 0; j < QW; ++j)
+    {
+        const int q = wv + 4 * j;
+        if (q < nrhs && lane < nr)
+            B[(size_t)(r0 + lane) * nrhs + q] = x[j];
     }
 }
 
