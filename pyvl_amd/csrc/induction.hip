@@ -256,12 +256,7 @@ __global__ void __launch_bounds__(kBlock)
     double vx = 0.0, vy = 0.0, vz = 0.0;
     for (unsigned l = a; l < b; ++l)
     {
-        const double g = gamma[l];
-        double kx, ky, kz;
-        line_k<PREC>(geom[l], px, py, pz, tol, tol2, kx, ky, kz);
-        vx = vx + g * kx;
-        vy = vy + g * ky;
-        vz = vz + g * kz;
+        line_k_acc<PREC>(geom[l], px, py, pz, tol, tol2, gamma[l], vx, vy, vz);
     }
     red[w][0][lane] = vx;
     red[w][1][lane] = vy;

@@ -8,7 +8,9 @@
 // compiled with -ffp-contract=off: the dot products, h^2 and the cut-off test below then
 // round exactly as the reference's GCC x86-64 build (left to right, no FMA), so the cut-off
 // branch is decided identically and the cancellation in h^2 near a segment's line (which
-// the reference's values carry) is reproduced.
+// the reference's values carry) is reproduced.  FMAs appear only where written explicitly:
+// in the angle evaluation and, for the non-reference forms, in the cross product and the
+// circulation-weighted accumulation (1-ulp differences, no effect on the cut-off).
 //
 // Angle forms:
 //   PVL_FP64_REF  the reference's expression: two FP64 atan2 calls of the HIP math library.
@@ -115,18 +117,19 @@ __device__ __forceinline__ double pvl_rsqrt(double x)
     return y;
 }
 
+// Scalar factor of K (angle / h; the cut-off is decided here) and dr1 = p - r1.
 template <int PREC>
-__device__ __forceinline__ void line_k(const LineGeom &g, double px, double py, double pz, double tol, double tol2,
-                                       double &kx, double &ky, double &kz)
+__device__ __forceinline__ double line_m(const LineGeom &g, double px, double py, double pz, double tol, double tol2,
+                                         double &d1x, double &d1y, double &d1z, bool &cut)
 {
-    const double d1x = px - g.r1x, d1y = py - g.r1y, d1z = pz - g.r1z;
+    d1x = px - g.r1x, d1y = py - g.r1y, d1z = pz - g.r1z;
     const double d2x = px - g.r2x, d2y = py - g.r2y, d2z = pz - g.r2z;
     const double t1 = g.ex * d1x + g.ey * d1y + g.ez * d1z;
     const double t2 = g.ex * d2x + g.ey * d2y + g.ez * d2z;
     const double n1 = (d1x * d1x + d1y * d1y + d1z * d1z) - t1 * t1;
     const double n2 = (d2x * d2x + d2y * d2y + d2z * d2z) - t2 * t2;
     const double h2 = (n1 + n2) / 2.0;
-    const bool cut = (g.len < tol) || (h2 < tol2);
+    cut = (g.len < tol) || (h2 < tol2);
     const double hh = cut ? 1.0 : h2;
     double m;
     if constexpr (PREC == PVL_FP64_REF_FORM)
@@ -149,10 +152,61 @@ __device__ __forceinline__ void line_k(const LineGeom &g, double px, double py, 
         const float den = (float)fma(t1, t2, hh);
         m = (double)(pvl_atan2f(num, den) * ihf);
     }
-    const double cx = d1y * g.ez - d1z * g.ey;
-    const double cy = d1z * g.ex - d1x * g.ez;
-    const double cz = d1x * g.ey - d1y * g.ex;
-    kx = cut ? 0.0 : cx * m;
-    ky = cut ? 0.0 : cy * m;
-    kz = cut ? 0.0 : cz * m;
+    return m;
+}
+
+// K(line, p).  The reference form keeps the reference's cross product and products; the
+// others fuse the cross product's second product into an FMA (the cut-off decision and
+// every quantity it depends on are unchanged).
+template <int PREC>
+__device__ __forceinline__ void line_k(const LineGeom &g, double px, double py, double pz, double tol, double tol2,
+                                       double &kx, double &ky, double &kz)
+{
+    double d1x, d1y, d1z;
+    bool cut;
+    const double m = line_m<PREC>(g, px, py, pz, tol, tol2, d1x, d1y, d1z, cut);
+    if constexpr (PREC == PVL_FP64_REF_FORM)
+    {
+        const double cx = d1y * g.ez - d1z * g.ey;
+        const double cy = d1z * g.ex - d1x * g.ez;
+        const double cz = d1x * g.ey - d1y * g.ex;
+        kx = cut ? 0.0 : cx * m;
+        ky = cut ? 0.0 : cy * m;
+        kz = cut ? 0.0 : cz * m;
+    }
+    else
+    {
+        const double cx = fma(d1y, g.ez, -(d1z * g.ey));
+        const double cy = fma(d1z, g.ex, -(d1x * g.ez));
+        const double cz = fma(d1x, g.ey, -(d1y * g.ex));
+        kx = cut ? 0.0 : cx * m;
+        ky = cut ? 0.0 : cy * m;
+        kz = cut ? 0.0 : cz * m;
+    }
+}
+
+// v += w K(line, p): the point-velocity inner step (w = the line's circulation).  The
+// non-reference forms scale once (w m) and accumulate with FMAs.
+template <int PREC>
+__device__ __forceinline__ void line_k_acc(const LineGeom &g, double px, double py, double pz, double tol,
+                                           double tol2, double w, double &vx, double &vy, double &vz)
+{
+    if constexpr (PREC == PVL_FP64_REF_FORM)
+    {
+        double kx, ky, kz;
+        line_k<PREC>(g, px, py, pz, tol, tol2, kx, ky, kz);
+        vx = vx + w * kx;
+        vy = vy + w * ky;
+        vz = vz + w * kz;
+    }
+    else
+    {
+        double d1x, d1y, d1z;
+        bool cut;
+        const double m = line_m<PREC>(g, px, py, pz, tol, tol2, d1x, d1y, d1z, cut);
+        const double wm = cut ? 0.0 : w * m;
+        vx = fma(wm, fma(d1y, g.ez, -(d1z * g.ey)), vx);
+        vy = fma(wm, fma(d1z, g.ex, -(d1x * g.ez)), vy);
+        vz = fma(wm, fma(d1x, g.ey, -(d1y * g.ex)), vz);
+    }
 }
