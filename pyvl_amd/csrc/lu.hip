@@ -534,6 +534,71 @@ __global__ void __launch_bounds__(256) trsv_step_kernel(const T *__restrict__ A,
     }
 }
 
+// Whole triangular solve in ONE workgroup (1024 threads) for small systems: x (n x nrhs)
+// lives in LDS; blocks of 64 rows are taken in solve order, each first reduced by the
+// already solved part (wave-per-row dot products over contiguous factor rows) and then
+// solved with its diagonal block by substitution (wave 0, LAPACK dtrsv order).  Used when
+// the LDS holds x (n * nrhs * sizeof(T) <= kTrsvWgBytes).
+constexpr int kTrsvWgBytes = 64 * 1024;
+template <typename T, bool LOWER>
+__global__ void __launch_bounds__(1024) trsv_wg_kernel(const T *__restrict__ A, int lda, int n, T *__restrict__ B,
+                                                      int nrhs)
+{
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    T *x = (T *)smem_raw; // n x nrhs
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    for (int i = tid; i < n * nrhs; i += 1024)
+        x[i] = B[i];
+    __syncthreads();
+    const int nblk = (n + NB - 1) / NB;
+    for (int t = 0; t < nblk; ++t)
+    {
+        const int ib = LOWER ? t : nblk - 1 - t;
+        const int r0 = ib * NB, nr = min(NB, n - r0);
+        // solved columns: [0, r0) for L, [r0 + nr, n) for U
+        const int c0 = LOWER ? 0 : r0 + nr, c1 = LOWER ? r0 : n;
+        for (int r = wv; r < nr; r += 16)
+        {
+            const T *arow = A + (size_t)(r0 + r) * lda;
+            for (int q = 0; q < nrhs; ++q)
+            {
+                T s2 = T(0);
+                for (int c = c0 + lane; c < c1; c += 64)
+                    s2 = fma(arow[c], x[c * nrhs + q], s2);
+                for (int off = 32; off > 0; off >>= 1)
+                    s2 += __shfl_xor(s2, off);
+                if (lane == 0)
+                    x[(r0 + r) * nrhs + q] -= s2;
+            }
+        }
+        __syncthreads();
+        if (wv == 0)
+        {
+            const T *drow = A + (size_t)(r0 + min(lane, nr - 1)) * lda + r0;
+            for (int q = 0; q < nrhs; ++q)
+            {
+                T v = lane < nr ? x[(r0 + lane) * nrhs + q] : T(0);
+                const T dd = lane < nr ? drow[min(lane, nr - 1)] : T(1);
+                for (int kk = 0; kk < nr; ++kk)
+                {
+                    const int k = LOWER ? kk : nr - 1 - kk;
+                    if (!LOWER && lane == k)
+                        v = v / dd;
+                    const T vk = bcast_lane(v, k);
+                    const T lk = lane < nr ? drow[k] : T(0);
+                    if (LOWER ? lane > k : lane < k)
+                        v = fma(-lk, vk, v);
+                }
+                if (lane < nr)
+                    x[(r0 + lane) * nrhs + q] = v;
+            }
+        }
+        __syncthreads();
+    }
+    for (int i = tid; i < n * nrhs; i += 1024)
+        B[i] = x[i];
+}
+
 // R = B - A X  (FP64; A n x n row-major; X, B, R n x nrhs).  One wave per row.
 template <typename TA>
 __global__ void __launch_bounds__(256) residual_kernel(const TA *__restrict__ A, int lda, int n,
@@ -773,6 +838,16 @@ template <typename T> int solve_impl(pvl_lu *h, T *B, int nrhs)
             return PVL_ERR_ARG;
         }
         PVL_HIP(hipGetLastError());
+        const size_t xbytes = sizeof(T) * (size_t)n * nq;
+        if (xbytes <= (size_t)kTrsvWgBytes)
+        {
+            ProfScope prof(sizeof(T) == 8 ? "lu_trsv_f64" : "lu_trsv_f32");
+            const T *Ap = (const T *)h->lu;
+            trsv_wg_kernel<T, true><<<1, 1024, xbytes, stream()>>>(Ap, n, n, work, nq);
+            trsv_wg_kernel<T, false><<<1, 1024, xbytes, stream()>>>(Ap, n, n, work, nq);
+            PVL_HIP(hipGetLastError());
+        }
+        else
         {
             ProfScope prof(sizeof(T) == 8 ? "lu_trsv_f64" : "lu_trsv_f32");
             const T *Ap = (const T *)h->lu;
