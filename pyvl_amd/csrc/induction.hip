@@ -31,8 +31,13 @@ constexpr int kCptTile = 64; // control points per workgroup (one per lane)
 // ---------------------------------------------------------------------------------------
 // Per-line prologue (src/core/flow_solver.c:95-103): end points, |d| and d/|d| with three
 // true divisions.
+// A line shorter than tol is cut off for every point (K = 0 in every form); its direction
+// is stored as 0 instead of the 0/0 of a zero-length line, so that the fused accumulation
+// (0 * direction) stays 0.  With tol <= 0 nothing is cut by length and the reference's
+// values, NaN included, are kept.
 __global__ void __launch_bounds__(256) line_geom_kernel(unsigned n_lines, const uint32_t *__restrict__ lines,
-                                                       const double *__restrict__ pos, LineGeom *__restrict__ out)
+                                                       const double *__restrict__ pos, double tol,
+                                                       LineGeom *__restrict__ out)
 {
     const unsigned i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n_lines)
@@ -46,6 +51,8 @@ __global__ void __launch_bounds__(256) line_geom_kernel(unsigned n_lines, const 
     g.ex = dx / g.len;
     g.ey = dy / g.len;
     g.ez = dz / g.len;
+    if (g.len < tol)
+        g.ex = g.ey = g.ez = 0.0;
     out[i] = g;
 }
 
@@ -372,23 +379,23 @@ __global__ void __launch_bounds__(256) surface_geometry_kernel(unsigned n_surf, 
 }
 
 // ---------------------------------------------------------------------------------------
-int compute_line_geom(const pvl_mesh *m, const double *pos, LineGeom **out, unsigned n_lines, int slot = 0)
+int compute_line_geom(const pvl_mesh *m, const double *pos, double tol, LineGeom **out, unsigned n_lines, int slot = 0)
 {
     LineGeom *g = (LineGeom *)scratch(slot, sizeof(LineGeom) * (size_t)(n_lines ? n_lines : 1));
     if (!g)
         return PVL_ERR_NOMEM;
     if (n_lines)
     {
-        line_geom_kernel<<<(n_lines + 255) / 256, 256, 0, stream()>>>(n_lines, m->d_lines, pos, g);
+        line_geom_kernel<<<(n_lines + 255) / 256, 256, 0, stream()>>>(n_lines, m->d_lines, pos, tol, g);
         PVL_HIP(hipGetLastError());
     }
     *out = g;
     return PVL_OK;
 }
 
-int compute_line_geom(const pvl_mesh *m, const double *pos, LineGeom **out)
+int compute_line_geom(const pvl_mesh *m, const double *pos, double tol, LineGeom **out)
 {
-    return compute_line_geom(m, pos, out, m->n_lines, 0);
+    return compute_line_geom(m, pos, tol, out, m->n_lines, 0);
 }
 
 int check_common(const pvl_mesh *m, double tol, int precision)
@@ -414,7 +421,7 @@ int launch_surface(const pvl_mesh *m, double tol, const double *pos, unsigned n_
     if (n_cpts == 0 || m->n_surfaces == 0)
         return PVL_OK;
     LineGeom *g;
-    if (int st = compute_line_geom(m, pos, &g))
+    if (int st = compute_line_geom(m, pos, tol, &g))
         return st;
     constexpr int spw = NORMAL ? 16 : 8;
     const dim3 grid((m->n_surfaces + 4 * spw - 1) / (4 * spw), (n_cpts + kCptTile - 1) / kCptTile);
@@ -455,7 +462,7 @@ int point_velocity_impl(const pvl_mesh *m, double tol, const double *pos, const 
         return PVL_OK;
     }
     LineGeom *g;
-    if (int st = compute_line_geom(m, pos, &g, n_active, 0))
+    if (int st = compute_line_geom(m, pos, tol, &g, n_active, 0))
         return st;
     double *gamma = (double *)scratch(1, sizeof(double) * n_active);
     if (!gamma)
@@ -525,7 +532,7 @@ int pvl_line_induction_matrix(const pvl_mesh *m, double tol, const double *pos, 
     if (n_cpts == 0 || m->n_lines == 0)
         return PVL_OK;
     LineGeom *g;
-    if (int st = compute_line_geom(m, pos, &g))
+    if (int st = compute_line_geom(m, pos, tol, &g))
         return st;
     constexpr int lpw = 8;
     const dim3 grid((m->n_lines + 4 * lpw - 1) / (4 * lpw), (n_cpts + kCptTile - 1) / kCptTile);
