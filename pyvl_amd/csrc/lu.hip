@@ -536,7 +536,7 @@ __global__ void __launch_bounds__(256) trsv_step_kernel(const T *__restrict__ A,
 
 // Whole triangular solve in ONE workgroup (1024 threads) for small systems: x (n x nrhs)
 // lives in LDS; blocks of 64 rows are taken in solve order, each first reduced by the
-// already solved part (wave-per-row dot products over contiguous factor rows) and then
+// already solved part (dot products over contiguous factor rows, 4 rows per wave) and then
 // solved with its diagonal block by substitution (wave 0, LAPACK dtrsv order).  Used when
 // the LDS holds x (n * nrhs * sizeof(T) <= kTrsvWgBytes).
 constexpr int kTrsvWgBytes = 64 * 1024;
@@ -557,18 +557,32 @@ __global__ void __launch_bounds__(1024) trsv_wg_kernel(const T *__restrict__ A, 
         const int r0 = ib * NB, nr = min(NB, n - r0);
         // solved columns: [0, r0) for L, [r0 + nr, n) for U
         const int c0 = LOWER ? 0 : r0 + nr, c1 = LOWER ? r0 : n;
-        for (int r = wv; r < nr; r += 16)
+        // 16 waves x 4 rows = the 64 rows of the block, the four rows' loads in flight together
         {
-            const T *arow = A + (size_t)(r0 + r) * lda;
+            const int ra = 4 * wv;
+            const T *arow[4];
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+                arow[j] = A + (size_t)(r0 + min(ra + j, nr - 1)) * lda;
             for (int q = 0; q < nrhs; ++q)
             {
-                T s2 = T(0);
+                T s4[4] = {T(0), T(0), T(0), T(0)};
                 for (int c = c0 + lane; c < c1; c += 64)
-                    s2 = fma(arow[c], x[c * nrhs + q], s2);
-                for (int off = 32; off > 0; off >>= 1)
-                    s2 += __shfl_xor(s2, off);
-                if (lane == 0)
-                    x[(r0 + r) * nrhs + q] -= s2;
+                {
+                    const T xv = x[c * nrhs + q];
+#pragma unroll
+                    for (int j = 0; j < 4; ++j)
+                        s4[j] = fma(arow[j][c], xv, s4[j]);
+                }
+#pragma unroll
+                for (int j = 0; j < 4; ++j)
+                {
+                    T s2 = s4[j];
+                    for (int off = 32; off > 0; off >>= 1)
+                        s2 += __shfl_xor(s2, off);
+                    if (lane == 0 && ra + j < nr)
+                        x[(r0 + ra + j) * nrhs + q] -= s2;
+                }
             }
         }
         __syncthreads();
