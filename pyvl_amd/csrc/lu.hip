@@ -546,6 +546,7 @@ __global__ void __launch_bounds__(1024) trsv_wg_kernel(const T *__restrict__ A, 
 {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     T *x = (T *)smem_raw; // n x nrhs
+    __shared__ T Ds[NB][NB + 1];
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     for (int i = tid; i < n * nrhs; i += 1024)
         x[i] = B[i];
@@ -586,22 +587,27 @@ __global__ void __launch_bounds__(1024) trsv_wg_kernel(const T *__restrict__ A, 
             }
         }
         __syncthreads();
+        // the diagonal block, staged in LDS by all threads (identity padding past n)
+        for (int idx = tid; idx < NB * NB; idx += 1024)
+        {
+            const int r = idx / NB, c = idx % NB;
+            Ds[r][c] = (r < nr && c < nr) ? A[(size_t)(r0 + r) * lda + r0 + c] : (r == c ? T(1) : T(0));
+        }
+        __syncthreads();
         if (wv == 0)
         {
-            const T *drow = A + (size_t)(r0 + min(lane, nr - 1)) * lda + r0;
+            const T dd = Ds[lane][lane];
             for (int q = 0; q < nrhs; ++q)
             {
                 T v = lane < nr ? x[(r0 + lane) * nrhs + q] : T(0);
-                const T dd = lane < nr ? drow[min(lane, nr - 1)] : T(1);
-                for (int kk = 0; kk < nr; ++kk)
+                for (int kk = 0; kk < NB; ++kk)
                 {
-                    const int k = LOWER ? kk : nr - 1 - kk;
+                    const int k = LOWER ? kk : NB - 1 - kk;
                     if (!LOWER && lane == k)
                         v = v / dd;
                     const T vk = bcast_lane(v, k);
-                    const T lk = lane < nr ? drow[k] : T(0);
                     if (LOWER ? lane > k : lane < k)
-                        v = fma(-lk, vk, v);
+                        v = fma(-Ds[lane][k], vk, v);
                 }
                 if (lane < nr)
                     x[(r0 + lane) * nrhs + q] = v;
