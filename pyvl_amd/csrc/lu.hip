@@ -568,6 +568,7 @@ __global__ void __launch_bounds__(1024) trsv_wg_kernel(const T *__restrict__ A, 
             for (int q = 0; q < nrhs; ++q)
             {
                 T s4[4] = {T(0), T(0), T(0), T(0)};
+#pragma unroll 8
                 for (int c = c0 + lane; c < c1; c += 64)
                 {
                     const T xv = x[c * nrhs + q];

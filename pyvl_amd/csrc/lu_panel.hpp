@@ -53,11 +53,11 @@ __global__ void __launch_bounds__(512) panel_wg_kernel(T *__restrict__ P, int ld
         for (int c = 0; c < NBP; ++c)
             a[s][c] = (r < m && c < jb) ? P[(size_t)r * lda + c] : T(0);
     }
-    for (int J = 0; J < jb; ++J)
-    {
-        const int par = J & 1;
-        // this thread's best candidate in the current column (slot 0) among rows >= J; rows
-        // ascend with s, so a tie keeps the first
+    // Wave-level argmax of column J (window slot 0) among rows >= J, left in red[J & 1].
+    // For J >= 1 it runs at the end of the previous column's step, in the same basic block
+    // as the tail of its rank-1 update, so the DPP chain overlaps those FMAs.
+    auto candidate = [&](int J) {
+        // rows ascend with s, so a tie keeps the first
         unsigned long long key = 0;
         unsigned row = NONE;
 #pragma unroll
@@ -75,7 +75,12 @@ __global__ void __launch_bounds__(512) panel_wg_kernel(T *__restrict__ P, int ld
         unsigned long long wk;
         const unsigned wr = wave_argmax(key, row, &wk);
         if (lane == 0)
-            red_k[par][w] = wk, red_p[par][w] = wr;
+            red_k[J & 1][w] = wk, red_p[J & 1][w] = wr;
+    };
+    candidate(0);
+    for (int J = 0; J < jb; ++J)
+    {
+        const int par = J & 1;
         __syncthreads();
         unsigned long long bk = 0;
         unsigned br = NONE;
@@ -150,6 +155,8 @@ __global__ void __launch_bounds__(512) panel_wg_kernel(T *__restrict__ P, int ld
                 a[s][NBP - 1] = l;
             }
         }
+        if (J + 1 < jb) // uniform
+            candidate(J + 1);
     }
     __syncthreads();
     if (tid < jb)
