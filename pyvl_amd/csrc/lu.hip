@@ -475,6 +475,7 @@ __global__ void __launch_bounds__(256) trsv_step_kernel(const T *__restrict__ A,
         {
             const int r = idx / nrhs, q = idx - r * nrhs;
             T s2 = acc[r][q];
+#pragma unroll 8
             for (int c = 0; c < NB; ++c)
                 s2 = fma(-Ls[r][c], xs[c][q], s2);
             acc[r][q] = s2;
