@@ -261,20 +261,8 @@ __global__ void __launch_bounds__(kBlock)
     const unsigned a = l0 + w * per_wave;
     const unsigned b = (a + per_wave) < l1 ? (a + per_wave) : l1;
     double vx = 0.0, vy = 0.0, vz = 0.0;
-    if constexpr (PREC == PVL_FP64_REF_FORM)
-    {
-        for (unsigned l = a; l < b; ++l)
-            line_k_acc<PREC>(geom[l], px, py, pz, tol, tol2, gamma[l], vx, vy, vz);
-    }
-    else
-    {
-        // endpoint terms carried from line to line (no previous line: NaN never matches)
-        const double nan = __builtin_nan("");
-        double prev[6] = {nan, nan, nan, nan, nan, nan};
-        EndTerms e1{0, 0, 0, 0}, e2{0, 0, 0, 0};
-        for (unsigned l = a; l < b; ++l)
-            line_k_acc_chain<PREC>(geom[l], px, py, pz, tol, tol2, gamma[l], vx, vy, vz, e1, e2, prev);
-    }
+    for (unsigned l = a; l < b; ++l)
+        line_k_acc<PREC>(geom[l], px, py, pz, tol, tol2, gamma[l], vx, vy, vz);
     red[w][0][lane] = vx;
     red[w][1][lane] = vy;
     red[w][2][lane] = vz;

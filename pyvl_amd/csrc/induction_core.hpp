@@ -211,79 +211,3 @@ __device__ __forceinline__ void line_k_acc(const LineGeom &g, double px, double 
     }
 }
 
-// Point-velocity accumulation over a run of lines that reuses the endpoint terms of the
-// previous line: in mesh and wake line orders a line's start (or end) is usually the
-// previous line's end (or start), and then dr = p - r and |dr|^2 are the same numbers, so
-// they are taken over instead of recomputed (the same expressions on the same operands:
-// bit-identical).  Endpoints are compared as bit patterns of wave-uniform values, so the
-// choice is a scalar branch.  Non-reference forms only.
-struct EndTerms
-{
-    double dx, dy, dz, q; // p - r and |p - r|^2 (reference order)
-};
-
-__device__ __forceinline__ unsigned long long dbits(double v) { return (unsigned long long)__double_as_longlong(v); }
-
-__device__ __forceinline__ bool same_point(double ax, double ay, double az, double bx, double by, double bz)
-{
-    return dbits(ax) == dbits(bx) && dbits(ay) == dbits(by) && dbits(az) == dbits(bz);
-}
-
-__device__ __forceinline__ EndTerms end_terms(double px, double py, double pz, double rx, double ry, double rz)
-{
-    EndTerms e;
-    e.dx = px - rx, e.dy = py - ry, e.dz = pz - rz;
-    e.q = e.dx * e.dx + e.dy * e.dy + e.dz * e.dz;
-    return e;
-}
-
-template <int PREC>
-__device__ __forceinline__ void line_k_acc_chain(const LineGeom &g, double px, double py, double pz, double tol,
-                                                 double tol2, double w, double &vx, double &vy, double &vz,
-                                                 EndTerms &e1, EndTerms &e2, double (&prev)[6])
-{
-    static_assert(PREC != PVL_FP64_REF_FORM, "the reference form keeps its own expression");
-    // new start terms: previous end, previous start, or computed
-    EndTerms n1, n2;
-    if (same_point(g.r1x, g.r1y, g.r1z, prev[3], prev[4], prev[5]))
-        n1 = e2;
-    else if (same_point(g.r1x, g.r1y, g.r1z, prev[0], prev[1], prev[2]))
-        n1 = e1;
-    else
-        n1 = end_terms(px, py, pz, g.r1x, g.r1y, g.r1z);
-    if (same_point(g.r2x, g.r2y, g.r2z, prev[3], prev[4], prev[5]))
-        n2 = e2;
-    else if (same_point(g.r2x, g.r2y, g.r2z, prev[0], prev[1], prev[2]))
-        n2 = e1;
-    else
-        n2 = end_terms(px, py, pz, g.r2x, g.r2y, g.r2z);
-    e1 = n1, e2 = n2;
-    prev[0] = g.r1x, prev[1] = g.r1y, prev[2] = g.r1z;
-    prev[3] = g.r2x, prev[4] = g.r2y, prev[5] = g.r2z;
-    // as line_m, with the endpoint terms given
-    const double t1 = g.ex * n1.dx + g.ey * n1.dy + g.ez * n1.dz;
-    const double t2 = g.ex * n2.dx + g.ey * n2.dy + g.ez * n2.dz;
-    const double h2 = ((n1.q - t1 * t1) + (n2.q - t2 * t2)) / 2.0;
-    const bool cut = (g.len < tol) || (h2 < tol2);
-    const double hh = cut ? 1.0 : h2;
-    double m;
-    if constexpr (PREC == PVL_FP64)
-    {
-        const double ih = pvl_rsqrt(hh);
-        const double h = hh * ih;
-        const double dphi = pvl_atan2((t2 - t1) * h, fma(t1, t2, hh));
-        m = dphi * ih;
-    }
-    else
-    {
-        const float hf = sqrtf((float)hh);
-        const float ihf = __builtin_amdgcn_rcpf(hf);
-        const float num = (float)(t2 - t1) * hf;
-        const float den = (float)fma(t1, t2, hh);
-        m = (double)(pvl_atan2f(num, den) * ihf);
-    }
-    const double wm = cut ? 0.0 : w * m;
-    vx = fma(wm, fma(n1.dy, g.ez, -(n1.dz * g.ey)), vx);
-    vy = fma(wm, fma(n1.dz, g.ex, -(n1.dx * g.ez)), vy);
-    vz = fma(wm, fma(n1.dx, g.ey, -(n1.dy * g.ex)), vz);
-}
