@@ -403,6 +403,9 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true", help="skip the host baselines")
     ap.add_argument("--headline-only", action="store_true", help="only the FP64 free-wake workload")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="first copy 1 GiB device-to-device (a plain copy kernel of known size, the reference "
+                         "for scaling hardware byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus}); "
@@ -411,6 +414,16 @@ def main() -> int:
     from pyvl_amd import _lib
 
     _lib.call("pvl_init", 0)
+    if args.copy_calibration:
+        from pyvl_amd import DeviceArray
+
+        src, dst = DeviceArray((1 << 27,)), DeviceArray((1 << 27,))  # 1 GiB each
+        _lib.call("pvl_memset_zero", src.cptr, C.c_size_t(src.nbytes))
+        _lib.call("pvl_memcpy_d2d", dst.cptr, src.cptr, C.c_size_t(src.nbytes))
+        _lib.call("pvl_synchronize")
+        log("copy calibration: 1 GiB device-to-device copy done")
+        src.free()
+        dst.free()
     head = run_c3("free", 0, args.steps, args.warmup, profile=True)
     extra = {"c3_grow_400_steps_s": round(head["grow_400_s"], 3)}
     extra["c3_kernel_breakdown_ms_per_step"] = head["breakdown"]
