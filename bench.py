@@ -36,6 +36,11 @@ sys.path.insert(0, str(ROOT))
 import numpy as np  # noqa: E402
 
 PEAK_FP64_TFLOPS = 78.6  # MI355X FP64 (vector and MFMA)
+# HBM bytes per launch of the full-wake self-induction kernel: rocprofv3 FETCH_SIZE and
+# WRITE_SIZE in two runs of this script (--copy-calibration), each scaled by what the same
+# counter reads for the script's own 1 GiB copy (FETCH x 2.0, WRITE x 1.0); the commands and
+# numbers are in profiles/wake_self_velocity_c3_fp64_traffic.md.
+SELF_VELOCITY_HBM_BYTES = 80.27e6
 PEAK_HBM_GBS = 8000.0
 FLOP_PER_PAIR = 87.0  # induction pair (src/core/flow_solver.c:114-135, atan2 counted as 20)
 FLOP_WEIGHT = 6.0  # Gamma weighting of a point-velocity sum
@@ -112,7 +117,7 @@ def roofline_from(prof) -> dict:
         "peak": PEAK_FP64_TFLOPS,
         "unit": "TFLOP/s",
         "frac": round(achieved / PEAK_FP64_TFLOPS, 4),
-        "traffic": None,
+        "traffic": SELF_VELOCITY_HBM_BYTES,
         "kernel": "point_velocity_kernel<FP64, wake-self> (free-wake self-induction)",
         "note": "FP64 vector-ALU bound (MI355X FP64 vector peak = FP64 MFMA peak = 78.6 TFLOP/s); "
                 f"algorithmic {FLOP_PER_PAIR + FLOP_WEIGHT:.0f} FLOP per (segment, point) pair, "
