@@ -90,14 +90,34 @@ __device__ __forceinline__ void xstore_i(int *p, int v)
     __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// Exchange area for the panel kernel (double buffered by column parity).  Each group
-// publishes its candidate record (|value|, position, row) and then raises its flag to the
-// column's sequence number; readers wait for every group's flag and then read all records
-// at once.
-constexpr int kPanelRecords = 4096; // G * jb records read speculatively per column
+// Grid barrier on a monotonically increasing counter; bounded: on timeout sets *err.
+__device__ __forceinline__ void grid_barrier(unsigned *counter, unsigned target, int *err)
+{
+    __threadfence();
+    __syncthreads();
+    if (threadIdx.x == 0)
+    {
+        __hip_atomic_fetch_add(counter, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        unsigned spins = 0;
+        while (__hip_atomic_load(counter, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < target)
+        {
+            __builtin_amdgcn_s_sleep(1);
+            if (++spins > kSpinLimit || __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+            {
+                __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                break;
+            }
+        }
+    }
+    __syncthreads();
+}
+
+
+// Exchange area for the panel kernel (double buffered by column parity).
 template <typename T> struct PanelX
 {
-    unsigned flag[2][kMaxGroups]; // sequence number of the column each group last published
+    unsigned counter; // grid-barrier arrivals, monotonic over one factorisation
+    int pad[3];
     double cval[2][kMaxGroups];
     int cpos[2][kMaxGroups];
     T crow[2][kMaxGroups][NB];
@@ -105,25 +125,20 @@ template <typename T> struct PanelX
 };
 
 // ---------------------------------------------------------------------------------------
-// Panel factorisation of P (m x jb, row stride lda), rows row_base.. of the matrix, by a
-// cooperative grid.  Group g owns panel positions [g*R, g*R+R) in LDS.  Per column j: local
-// argmax over positions >= j; publish (|value|, position, row) (group 0 also the current
-// row j), then the flag = seq0 + j; wait until every group's flag shows it; read every
-// group's record and row in one round (G * jb <= kPanelRecords) and pick the same winner
-// in every group (largest |a|, first position on ties: LAPACK idamax); swap, scale by the
-// reciprocal pivot and apply the rank-1 update to the owned rows.  A column thus costs one
-// flag round trip and one read round trip.  Waits are bounded (status[1] on timeout).
+// Panel factorisation of P (m x jb, row stride lda), rows k.. of the matrix.  Group g owns
+// panel positions [g*R, g*R+R).  Per column j: local argmax over positions >= j, publish
+// (|value|, position, row) plus the current row j, grid barrier, every group picks the
+// same winner (largest |a|, first position on ties: LAPACK idamax), swaps, scales by the
+// reciprocal pivot and applies the rank-1 update to its rows.
 template <typename T>
 __global__ void __launch_bounds__(256) panel_kernel(T *__restrict__ P, int lda, int m, int jb, int R, int row_base,
-                                                    int *__restrict__ piv, PanelX<T> *__restrict__ X, unsigned seq0,
+                                                    int *__restrict__ piv, PanelX<T> *__restrict__ X, unsigned base,
                                                     int *__restrict__ status)
 {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     constexpr int LDP = NB + 1;
-    constexpr int PER_T = kPanelRecords / 256; // speculative row entries per thread
     T *L = (T *)smem_raw;
     __shared__ T prow[NB];
-    __shared__ T srowj[NB];
     __shared__ double red_v[4];
     __shared__ int red_p[4];
     __shared__ int s_pos, s_grp;
@@ -145,7 +160,6 @@ __global__ void __launch_bounds__(256) panel_kernel(T *__restrict__ P, int lda, 
     for (int j = 0; j < jb; ++j)
     {
         const int par = j & 1;
-        const unsigned seq = seq0 + (unsigned)j;
         // 1. local candidate
         double bv = -1.0;
         int bp = INT_MAX;
@@ -173,7 +187,7 @@ __global__ void __launch_bounds__(256) panel_kernel(T *__restrict__ P, int lda, 
         for (int w = 1; w < 4; ++w)
             if (better(red_v[w], red_p[w], bv, bp))
                 bv = red_v[w], bp = red_p[w];
-        // 2. publish the record (group 0 also the current row j), then the flag
+        // 2. publish candidate, its row, and the current row j if owned
         if (tid == 0)
         {
             xstore(&X->cval[par][g], bv);
@@ -186,41 +200,12 @@ __global__ void __launch_bounds__(256) panel_kernel(T *__restrict__ P, int lda, 
         if (own_j)
             for (int c = tid; c < jb; c += 256)
                 xstore(&X->rowj[par][c], L[(j - r0) * LDP + c]);
-        __threadfence();
-        __syncthreads();
-        if (tid == 0)
-            __hip_atomic_store(&X->flag[par][g], seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-        // 3. wait for every group's flag (thread q watches group q)
-        for (int q = tid; q < G; q += 256)
-        {
-            unsigned spins = 0;
-            // flags only grow within a factorisation (a group can be at most one column of the
-            // same parity ahead only after every group has passed this wait)
-            while (__hip_atomic_load(&X->flag[par][q], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < seq)
-            {
-                __builtin_amdgcn_s_sleep(1);
-                if (++spins > kSpinLimit || __hip_atomic_load(&status[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
-                {
-                    __hip_atomic_store(&status[1], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                    break;
-                }
-            }
-        }
-        __syncthreads();
-        // 4. one read round: every record, every candidate row (when they fit) and row j
-        const bool all_rows = G * jb <= kPanelRecords;
-        T spec[PER_T];
-        if (all_rows)
-        {
-#pragma unroll
-            for (int k = 0; k < PER_T; ++k)
-            {
-                const int idx = tid + 256 * k;
-                spec[k] = idx < G * jb ? xload(&X->crow[par][idx / jb][idx % jb]) : T(0);
-            }
-        }
-        for (int c = tid; c < jb; c += 256)
-            srowj[c] = xload(&X->rowj[par][c]);
+        // 3. barrier
+        if (G > 1)
+            grid_barrier(&X->counter, base + (unsigned)G * (unsigned)(j + 1), &status[1]);
+        else
+            __syncthreads();
+        // 4. winner (identical in every group)
         if (tid < 64)
         {
             double v = -1.0;
@@ -249,26 +234,8 @@ __global__ void __launch_bounds__(256) panel_kernel(T *__restrict__ P, int lda, 
         const bool none = (p == INT_MAX); // every candidate NaN: keep row j in place
         if (none)
             p = j;
-        if (none)
-        {
-            for (int c = tid; c < jb; c += 256)
-                prow[c] = srowj[c];
-        }
-        else if (all_rows)
-        {
-#pragma unroll
-            for (int k = 0; k < PER_T; ++k)
-            {
-                const int idx = tid + 256 * k;
-                if (idx < G * jb && idx / jb == wg)
-                    prow[idx % jb] = spec[k];
-            }
-        }
-        else
-        {
-            for (int c = tid; c < jb; c += 256)
-                prow[c] = xload(&X->crow[par][wg][c]);
-        }
+        for (int c = tid; c < jb; c += 256)
+            prow[c] = none ? xload(&X->rowj[par][c]) : xload(&X->crow[par][wg][c]);
         __syncthreads();
         // 5. interchange
         const bool own_p = p >= r0 && p < r0 + nr;
@@ -279,7 +246,7 @@ __global__ void __launch_bounds__(256) panel_kernel(T *__restrict__ P, int lda, 
                     L[(j - r0) * LDP + c] = prow[c];
             if (own_p)
                 for (int c = tid; c < jb; c += 256)
-                    L[(p - r0) * LDP + c] = srowj[c];
+                    L[(p - r0) * LDP + c] = xload(&X->rowj[par][c]);
         }
         if (tid == 0)
             spiv[j] = row_base + p; // global store deferred: see the end of the kernel
@@ -767,9 +734,9 @@ template <typename T> int factor_impl(int n, T *A, int *piv, int *info_out)
     }
     PanelX<T> *X = (PanelX<T> *)g_ws.panelx;
     int *status = g_ws.err;
-    // flags restart at 0; column k + j of this factorisation publishes sequence k + j + 1
-    PVL_HIP(hipMemsetAsync(X->flag, 0, sizeof(X->flag), stream()));
+    PVL_HIP(hipMemsetAsync(X, 0, sizeof(unsigned), stream()));
     PVL_HIP(hipMemsetAsync(status, 0, 2 * sizeof(int), stream()));
+    unsigned base = 0;
     // Panel width: 32 up to n = 2048, where single-workgroup register panels cover every
     // panel and the trailing updates are latency-bound anyway; 64 above.
     const int nbf = n <= 2048 ? kPanelWgCols : NB;
@@ -797,8 +764,10 @@ template <typename T> int factor_impl(int n, T *A, int *piv, int *info_out)
             T *P = A + (size_t)k * n + k;
             int lda = n, row_base = k;
             int mm = m, jbb = jb, RR = R;
-            unsigned seq0 = (unsigned)k + 1u;
-            void *args[] = {&P, &lda, &mm, &jbb, &RR, &row_base, &piv, &X, &seq0, &status};
+            unsigned bb = base;
+            void *args[] = {&P, &lda, &mm, &jbb, &RR, &row_base, &piv, &X, &bb, &status};
+            if (G > 1)
+                base += (unsigned)G * (unsigned)jb;
             const size_t lds_now = sizeof(T) * (size_t)R * (NB + 1);
             {
                 ProfScope prof(sizeof(T) == 8 ? "lu_panel_f64" : "lu_panel_f32");
