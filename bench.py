@@ -413,8 +413,8 @@ def main() -> int:
                          "for scaling hardware byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus}); "
-              "the way the workload would shard is described in DESIGN.md", file=sys.stderr)
+        print(f"bench.py measures one GPU: there is no multi-GPU path (asked for --gpus {args.gpus}); "
+              "the workload would shard by evaluation point, which is a design note, not code", file=sys.stderr)
         return 2
     from pyvl_amd import _lib
 
