@@ -318,6 +318,18 @@ def run_c4(do_cpu: bool) -> dict:
     t32 = time.perf_counter() - t0
     x32 = mean_free(X32.numpy())
     log(f"C4 FP32 LU + FP64 IR (16 RHS): {t32 * 1e3:.1f} ms, {iters} refinement steps")
+    # FP32 LU without pivoting (no per-column row search) + FP64 refinement
+    lu32n = LA.LU(n, LA.FP32, pivoting=False)
+    lu32n.factor(A)
+    X32n = DeviceArray((n, 16))
+    _lib.call("pvl_synchronize")
+    t0 = time.perf_counter()
+    lu32n.factor(A)
+    _, iters_n, _, relres_n = LA.solve_refine(A, dB, lu32n, tol=1e-12, out=X32n)
+    _lib.call("pvl_synchronize")
+    t32n = time.perf_counter() - t0
+    x32n = mean_free(X32n.numpy())
+    log(f"C4 FP32 LU without pivoting + FP64 IR (16 RHS): {t32n * 1e3:.1f} ms, {iters_n} refinement steps")
     bmax = np.max(np.abs(B), axis=0)
     res64 = float(np.max(np.max(np.abs(LA.residual(A, X64, dB).numpy()), axis=0) / bmax))
     rel = lambda a, b: float(np.max(np.linalg.norm(a - b, axis=0) / np.linalg.norm(b, axis=0)))  # noqa: E731
@@ -332,22 +344,29 @@ def run_c4(do_cpu: bool) -> dict:
         "ir_final_rel_residual": float(np.max(relres)),
         "fp64_lu_rel_residual": res64,
         "rel_err_fp32ir_vs_fp64gpu": rel(x32, x64),
+        "fp32_mfma_lu_nopiv_ir_s": round(t32n, 4),
+        "ir_nopiv_corrections": iters_n,
+        "ir_nopiv_final_rel_residual": float(np.max(relres_n)),
+        "rel_err_fp32nopiv_ir_vs_fp64gpu": rel(x32n, x64),
         "deflation_c": c,
         "speedup_fp32ir_over_fp64": round(t64 / t32, 3),
+        "speedup_fp32nopiv_ir_over_fp64": round(t64 / t32n, 3),
     }
     if do_cpu:
         import scipy.linalg as la
 
         Ad = A.numpy()
         t0 = time.perf_counter()
-        xr = la.lu_solve(la.lu_factor(Ad, overwrite_a=True, check_finite=False), B, check_finite=False)
+        xr_raw = la.lu_solve(la.lu_factor(Ad.copy(), overwrite_a=True, check_finite=False), B, check_finite=False)
         tcpu = time.perf_counter() - t0
-        xr = mean_free(xr)
+        xr = mean_free(xr_raw)
         out.update({
             "cpu_lapack_fp64_s": round(tcpu, 3),
             "cpu_cores": os.cpu_count(),
             "rel_err_fp64gpu_vs_lapack": rel(x64, xr),
             "rel_err_fp32ir_vs_lapack": rel(x32, xr),
+            "rel_err_fp32nopiv_ir_vs_lapack": rel(x32n, xr),
+            "lapack_rel_residual": float(np.max(np.max(np.abs(B - Ad @ xr_raw), axis=0) / bmax)),
         })
         log(f"C4 scipy LAPACK FP64 on {os.cpu_count()} cores: {tcpu:.2f} s")
     return out

@@ -162,6 +162,9 @@ typedef struct pvl_lu pvl_lu;
 /* precision PVL_FP64: factors kept in FP64; PVL_FP32: factors kept in FP32 (MFMA FP32). */
 int pvl_lu_create(pvl_lu **out, int n, int precision);
 int pvl_lu_destroy(pvl_lu *h);
+/* pivoting 1 (default): partial pivoting, LAPACK's interchanges; 0: none (an FP32
+ * factorisation for FP64 refinement on diagonally strong systems; no reference counterpart). */
+int pvl_lu_set_pivoting(pvl_lu *h, int pivoting);
 /* Copy A (FP64, device) into the handle (converting for FP32) and factor it.  Returns
  * PVL_ERR_SINGULAR with *info_out = first zero pivot (1-based) when U is exactly singular. */
 int pvl_lu_factor(pvl_lu *h, const double *A_dev, int *info_out);

@@ -91,6 +91,7 @@ SIGNATURES: dict[str, tuple] = {
         [C.c_void_p, C.c_double, _hp, C.c_uint, _hp, _hp, C.c_int],
     ),
     "pvl_lu_create": (C.c_int, [C.POINTER(C.c_void_p), C.c_int, C.c_int]),
+    "pvl_lu_set_pivoting": (C.c_int, [C.c_void_p, C.c_int]),
     "pvl_lu_destroy": (C.c_int, [C.c_void_p]),
     "pvl_lu_factor": (C.c_int, [C.c_void_p, _dp, C.POINTER(C.c_int)]),
     "pvl_lu_solve": (C.c_int, [C.c_void_p, _dp, C.c_int]),

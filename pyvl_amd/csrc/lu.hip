@@ -288,6 +288,86 @@ __global__ void __launch_bounds__(256) panel_kernel(T *__restrict__ P, int lda, 
     }
 }
 
+// Panel factorisation WITHOUT pivoting (for an FP32 factorisation ahead of FP64 refinement on
+// diagonally strong systems): every workgroup factors the jb x jb diagonal block of the
+// panel in LDS (column sweep, the same arithmetic everywhere, so the copies agree);
+// workgroup 0 stores L11 / U11 and the identity interchanges, workgroup g >= 1 solves its
+// 64 rows of the panel below the diagonal block, X U11 = A21 (column sweep), and stores
+// L21.  No workgroup waits for another.  status[0]: the first exactly-zero pivot.
+template <typename T>
+__global__ void __launch_bounds__(256) panel_nopiv_kernel(T *__restrict__ P, int lda, int m, int jb, int row_base,
+                                                          int *__restrict__ piv, int *__restrict__ status)
+{
+    __shared__ T D[NB][NB + 1];
+    __shared__ T X[NB][NB + 1];
+    const int tid = threadIdx.x, g = blockIdx.x;
+    for (int idx = tid; idx < NB * NB; idx += 256)
+    {
+        const int r = idx / NB, c = idx % NB;
+        D[r][c] = (r < jb && c < jb) ? P[(size_t)r * lda + c] : (r == c ? T(1) : T(0));
+    }
+    __syncthreads();
+    // D = L11 U11 (right-looking, no interchanges): thread (r, c-group)
+    for (int j = 0; j < jb - 1; ++j)
+    {
+        const T inv = T(1) / D[j][j];
+        for (int idx = tid; idx < NB * NB; idx += 256)
+        {
+            const int r = idx / NB, c = idx % NB;
+            if (r > j && r < jb && c > j && c < jb)
+                D[r][c] = fma(-(D[r][j] * inv), D[j][c], D[r][c]);
+        }
+        __syncthreads();
+        for (int r = j + 1 + tid; r < jb; r += 256)
+            D[r][j] = D[r][j] * inv;
+        __syncthreads();
+    }
+    if (g == 0)
+    {
+        for (int idx = tid; idx < jb * jb; idx += 256)
+        {
+            const int r = idx / jb, c = idx % jb;
+            P[(size_t)r * lda + c] = D[r][c];
+        }
+        for (int j = tid; j < jb; j += 256)
+        {
+            piv[row_base + j] = row_base + j;
+            if (D[j][j] == T(0))
+                atomicMin(&status[2], row_base + j + 1); // smallest zero-pivot column
+        }
+        return;
+    }
+    // rows r0 .. r0 + 63 of the panel below the diagonal block
+    const int r0 = jb + (g - 1) * NB;
+    const int nr = min(NB, m - r0);
+    for (int idx = tid; idx < NB * NB; idx += 256)
+    {
+        const int r = idx / NB, c = idx % NB;
+        X[r][c] = (r < nr && c < jb) ? P[(size_t)(r0 + r) * lda + c] : T(0);
+    }
+    __syncthreads();
+    // X U11 = A21: column c final once the earlier columns are folded in
+    for (int c = 0; c < jb; ++c)
+    {
+        const T inv = T(1) / D[c][c];
+        for (int r = tid; r < nr; r += 256)
+            X[r][c] = X[r][c] * inv;
+        __syncthreads();
+        for (int idx = tid; idx < NB * NB; idx += 256)
+        {
+            const int r = idx / NB, cc = idx % NB;
+            if (r < nr && cc > c && cc < jb)
+                X[r][cc] = fma(-X[r][c], D[c][cc], X[r][cc]);
+        }
+        __syncthreads();
+    }
+    for (int idx = tid; idx < nr * jb; idx += 256)
+    {
+        const int r = idx / jb, c = idx % jb;
+        P[(size_t)(r0 + r) * lda + c] = X[r][c];
+    }
+}
+
 // Row interchanges of one panel applied to every column outside it.
 template <typename T>
 __global__ void __launch_bounds__(256) laswp_kernel(T *__restrict__ A, int lda, int n, int k, int jb,
@@ -702,7 +782,7 @@ __global__ void add_f32_kernel(double *__restrict__ X, const float *__restrict__
 struct Workspace
 {
     void *panelx = nullptr;
-    int *err = nullptr; // err[0]: first zero pivot (factor), err[1]: a bounded wait timed out
+    int *err = nullptr; // [0]: first zero pivot, [1]: a bounded wait timed out, [2]: no-pivot zero pivot
 };
 Workspace g_ws;
 
@@ -711,15 +791,15 @@ int ensure_ws()
     if (!g_ws.panelx)
     {
         PVL_HIP(hipMalloc(&g_ws.panelx, sizeof(PanelX<double>)));
-        PVL_HIP(hipMalloc(&g_ws.err, 2 * sizeof(int)));
-        PVL_HIP(hipMemset(g_ws.err, 0, 2 * sizeof(int)));
+        PVL_HIP(hipMalloc(&g_ws.err, 4 * sizeof(int)));
+        PVL_HIP(hipMemset(g_ws.err, 0, 4 * sizeof(int)));
     }
     return PVL_OK;
 }
 
 template <typename T> int panel_rows_max() { return sizeof(T) == 8 ? 256 : 512; }
 
-template <typename T> int factor_impl(int n, T *A, int *piv, int *info_out)
+template <typename T> int factor_impl(int n, T *A, int *piv, int *info_out, bool pivoting)
 {
     if (int st = ensure_ws())
         return st;
@@ -739,12 +819,20 @@ template <typename T> int factor_impl(int n, T *A, int *piv, int *info_out)
     unsigned base = 0;
     // Panel width: 32 up to n = 2048, where single-workgroup register panels cover every
     // panel and the trailing updates are latency-bound anyway; 64 above.
-    const int nbf = n <= 2048 ? kPanelWgCols : NB;
+    PVL_HIP(hipMemsetAsync(status + 2, 0x7F, sizeof(int), stream())); // no-pivot zero pivots: min
+    const int nbf = !pivoting ? NB : (n <= 2048 ? kPanelWgCols : NB);
     for (int k = 0; k < n; k += nbf)
     {
         const int jb = std::min(nbf, n - k);
         const int m = n - k;
-        if (nbf == kPanelWgCols && m <= panel_wg_rows(sizeof(T)))
+        if (!pivoting)
+        {
+            T *P = A + (size_t)k * n + k;
+            ProfScope prof(sizeof(T) == 8 ? "lu_panel_f64" : "lu_panel_f32");
+            panel_nopiv_kernel<T><<<1 + (m - jb + NB - 1) / NB, 256, 0, stream()>>>(P, n, m, jb, k, piv, status);
+            PVL_HIP(hipGetLastError());
+        }
+        else if (nbf == kPanelWgCols && m <= panel_wg_rows(sizeof(T)))
         {
             T *P = A + (size_t)k * n + k;
             ProfScope prof(sizeof(T) == 8 ? "lu_panel_f64" : "lu_panel_f32");
@@ -776,6 +864,7 @@ template <typename T> int factor_impl(int n, T *A, int *piv, int *info_out)
             }
         }
         // info / err readback is deferred to the end (one sync per factorisation)
+        if (pivoting)
         {
             ProfScope prof(sizeof(T) == 8 ? "lu_laswp_f64" : "lu_laswp_f32");
             laswp_kernel<T><<<(n + 255) / 256, 256, 0, stream()>>>(A, n, n, k, jb, piv);
@@ -795,7 +884,7 @@ template <typename T> int factor_impl(int n, T *A, int *piv, int *info_out)
             PVL_HIP(hipGetLastError());
         }
     }
-    int st2[2];
+    int st2[3];
     PVL_HIP(hipMemcpyAsync(st2, status, sizeof(st2), hipMemcpyDeviceToHost, stream()));
     PVL_HIP(hipStreamSynchronize(stream()));
     if (st2[1])
@@ -803,7 +892,7 @@ template <typename T> int factor_impl(int n, T *A, int *piv, int *info_out)
         set_error("LU panel barrier timed out");
         return PVL_ERR_HIP;
     }
-    *info_out = st2[0];
+    *info_out = pivoting ? st2[0] : (st2[2] == 0x7F7F7F7F ? 0 : st2[2]);
     return PVL_OK;
 }
 
@@ -819,6 +908,7 @@ struct pvl_lu
     int *piv = nullptr;    // device, n
     int *perm = nullptr;   // device, n: row i of P B is row perm[i] of B
     void *work = nullptr;  // n * kMaxRhs (T) staging for permuted right-hand sides
+    bool pivoting = true;  // partial pivoting (LAPACK) or none
 };
 
 namespace
@@ -944,6 +1034,21 @@ int pvl_lu_destroy(pvl_lu *h)
     return PVL_OK;
 }
 
+// Partial pivoting (default; LAPACK's interchanges) or none: the unpivoted factorisation
+// has no row search, so its panels need no grid-wide step per column; it is meant for an
+// FP32 factorisation that FP64 refinement corrects, on systems whose diagonal dominates
+// enough (refinement reports PVL_ERR_NOTCONV otherwise).
+int pvl_lu_set_pivoting(pvl_lu *h, int pivoting)
+{
+    if (!h)
+    {
+        set_error("null LU handle");
+        return PVL_ERR_ARG;
+    }
+    h->pivoting = pivoting != 0;
+    return PVL_OK;
+}
+
 // Factor: copies A (n x n row-major, FP64 on the device) into the handle's storage in the
 // handle's precision and factors it.  info_out = first zero pivot (1-based) or 0.
 int pvl_lu_factor(pvl_lu *h, const double *A_dev, int *info_out)
@@ -956,7 +1061,7 @@ int pvl_lu_factor(pvl_lu *h, const double *A_dev, int *info_out)
     if (h->prec == PVL_FP64)
     {
         PVL_HIP(hipMemcpyAsync(h->lu, A_dev, sizeof(double) * cnt, hipMemcpyDeviceToDevice, stream()));
-        st = factor_impl<double>(n, (double *)h->lu, h->piv, &info);
+        st = factor_impl<double>(n, (double *)h->lu, h->piv, &info, h->pivoting);
         if (!st)
             st = finish_factor<double>(h);
     }
@@ -964,7 +1069,7 @@ int pvl_lu_factor(pvl_lu *h, const double *A_dev, int *info_out)
     {
         f64_to_f32_kernel<<<2048, 256, 0, stream()>>>(A_dev, (float *)h->lu, cnt);
         PVL_HIP(hipGetLastError());
-        st = factor_impl<float>(n, (float *)h->lu, h->piv, &info);
+        st = factor_impl<float>(n, (float *)h->lu, h->piv, &info, h->pivoting);
         if (!st)
             st = finish_factor<float>(h);
     }

@@ -25,14 +25,21 @@ FP32 = _lib.PVL_FP32
 
 
 class LU:
-    """Device LU factorisation of an n x n matrix (row-major), FP64 or FP32 factors."""
+    """Device LU factorisation of an n x n matrix (row-major), FP64 or FP32 factors.
 
-    def __init__(self, n: int, precision: int = FP64):
+    ``pivoting=False`` factors without row interchanges: meant for an FP32 factorisation
+    that FP64 refinement (:func:`solve_refine`) corrects, on diagonally strong systems such
+    as the deflated closed-body ones; the default is LAPACK's partial pivoting."""
+
+    def __init__(self, n: int, precision: int = FP64, pivoting: bool = True):
         self.n = int(n)
         self.precision = int(precision)
+        self.pivoting = bool(pivoting)
         h = C.c_void_p()
         _lib.call("pvl_lu_create", C.byref(h), C.c_int(self.n), C.c_int(self.precision))
         self._h = h
+        if not self.pivoting:
+            _lib.call("pvl_lu_set_pivoting", h, C.c_int(0))
         self.info = 0
 
     def factor(self, a) -> "LU":

@@ -173,3 +173,49 @@ def test_fp64_solve_backward_error_closed_body():
     rr = lambda v: np.max(np.abs(b - ad @ v)) / np.max(np.abs(b))  # noqa: E731
     assert rr(x) <= 10.0 * rr(x_ref) + 1e-14, (rr(x), rr(x_ref))
     assert _rel(x, x_ref) < 1e-11
+
+
+def _lu_nopiv(a):
+    """Unpivoted Doolittle LU (float64 numpy, tests only): packed L\\U."""
+    f = np.array(a, np.float64)
+    for j in range(f.shape[0] - 1):
+        f[j + 1:, j] /= f[j, j]
+        f[j + 1:, j + 1:] -= np.outer(f[j + 1:, j], f[j, j + 1:])
+    return f
+
+
+@pytest.mark.parametrize("n", [50, 64, 300, 2100])
+def test_unpivoted_factor(n):
+    """Without pivoting the FP64 factors equal the unpivoted LU and the pivots are the
+    identity (a diagonally dominant matrix, where the unpivoted LU is stable)."""
+    rng = np.random.default_rng(40 + n)
+    a = rng.normal(size=(n, n)) + 2.0 * n ** 0.5 * np.eye(n)
+    h = LA.LU(n, LA.FP64, pivoting=False).factor(a)
+    assert np.array_equal(h.pivots(), np.arange(n))
+    assert _rel(h.factors(), _lu_nopiv(a)) < 1e-12
+    b = rng.normal(size=n)
+    assert _rel(h.solve(b), np.linalg.solve(a, b)) < 1e-12
+
+
+def test_unpivoted_fp32_refinement_closed_body():
+    """FP32 LU without pivoting + FP64 refinement on the deflated closed-body system
+    (ellipsoid 4:1:0.5, 4,096 panels, 4 right-hand sides): the refined answer agrees with
+    LAPACK's FP64 solve to 1e-10 after a few corrections."""
+    pos, el = G.ellipsoid(64, 64, axes=(4.0, 1.0, 0.5), sigma_rel=1e-3, seed=0)
+    m = Mesh(pos.shape[0], el)
+    dp = DeviceArray.from_numpy(pos)
+    nrm = m.surface_normal(dp)
+    cps = m.surface_average_vec3(dp)
+    n = m.n_surfaces
+    a = DeviceArray((n, n))
+    m.induction_matrix3(1e-6, dp, cps, nrm, a)
+    LA.deflate(a, 0, n, LA.deflation_constant(np.diag(a.numpy())))
+    ad = a.numpy()
+    dirs = np.array([[1.0, 0.0, 0.0], [0.99, 0.1, 0.0], [0.99, 0.0, 0.1], [0.98, 0.1, 0.1]])
+    b = np.ascontiguousarray(-(nrm.numpy() @ dirs.T))
+    lu32 = LA.LU(n, LA.FP32, pivoting=False).factor(a)
+    x, iters, _, res = LA.solve_refine(a, DeviceArray.from_numpy(b), lu32, tol=1e-12)
+    assert iters <= 6 and np.all(res <= 1e-12)
+    x_ref = sla.lu_solve(sla.lu_factor(ad), b)
+    for q in range(b.shape[1]):
+        assert _rel(x.numpy()[:, q], x_ref[:, q]) < 1e-10
