@@ -551,14 +551,17 @@ __global__ void __launch_bounds__(256) trsv_step_kernel(const T *__restrict__ A,
             xs[r][q] = (r < nc) ? B[(size_t)(c0 + r) * nrhs + q] : T(0);
         }
         __syncthreads();
+        // the block's contribution as a dot product, subtracted once (as a blocked dtrsv
+        // with a GEMV update does; folding each term into the running value loses accuracy
+        // over hundreds of blocks)
         for (int idx = tid; idx < NB * nrhs; idx += 256)
         {
             const int r = idx / nrhs, q = idx - r * nrhs;
-            T s2 = acc[r][q];
+            T s2 = T(0);
 #pragma unroll 8
             for (int c = 0; c < NB; ++c)
-                s2 = fma(-Ls[r][c], xs[c][q], s2);
-            acc[r][q] = s2;
+                s2 = fma(Ls[r][c], xs[c][q], s2);
+            acc[r][q] = acc[r][q] - s2;
         }
         __syncthreads();
     }
