@@ -162,8 +162,20 @@ typedef struct pvl_lu pvl_lu;
 /* precision PVL_FP64: factors kept in FP64; PVL_FP32: factors kept in FP32 (MFMA FP32). */
 int pvl_lu_create(pvl_lu **out, int n, int precision);
 int pvl_lu_destroy(pvl_lu *h);
-/* pivoting 1 (default): partial pivoting, LAPACK's interchanges; 0: none (an FP32
- * factorisation for FP64 refinement on diagonally strong systems; no reference counterpart). */
+/* Pivoting of the factorisation:
+ *   PVL_PIVOT_PARTIAL (default): partial pivoting, LAPACK's interchanges;
+ *   PVL_PIVOT_AUTO: the same factorisation, computed without interchanges when partial
+ *     pivoting would make none (checked at every column's elimination: no |a_ij| above
+ *     |a_jj|), else refactored with partial pivoting -- the case of panel-method systems
+ *     whose diagonal dominates (thin lifting surfaces);
+ *   PVL_PIVOT_NONE: no interchanges at all (an FP32 factorisation for FP64 refinement on
+ *     diagonally strong systems).  No reference counterpart for AUTO and NONE. */
+enum pvl_pivoting
+{
+    PVL_PIVOT_NONE = 0,
+    PVL_PIVOT_PARTIAL = 1,
+    PVL_PIVOT_AUTO = 2
+};
 int pvl_lu_set_pivoting(pvl_lu *h, int pivoting);
 /* Copy A (FP64, device) into the handle (converting for FP32) and factor it.  Returns
  * PVL_ERR_SINGULAR with *info_out = first zero pivot (1-based) when U is exactly singular. */

@@ -307,9 +307,18 @@ __global__ void __launch_bounds__(256) panel_nopiv_kernel(T *__restrict__ P, int
         D[r][c] = (r < jb && c < jb) ? P[(size_t)r * lda + c] : (r == c ? T(1) : T(0));
     }
     __syncthreads();
-    // D = L11 U11 (right-looking, no interchanges): thread (r, c-group)
+    // D = L11 U11 (right-looking, no interchanges).  status[3] records whether partial
+    // pivoting would have interchanged: some |a_rj| > |a_jj| (or a zero / NaN pivot) at the
+    // column's elimination.
     for (int j = 0; j < jb - 1; ++j)
     {
+        if (g == 0)
+        {
+            const double pa = fabs((double)D[j][j]);
+            for (int r = j + 1 + tid; r < jb; r += 256)
+                if (!(fabs((double)D[r][j]) <= pa) || !(pa > 0.0))
+                    atomicOr(&status[3], 1);
+        }
         const T inv = T(1) / D[j][j];
         for (int idx = tid; idx < NB * NB; idx += 256)
         {
@@ -350,8 +359,13 @@ __global__ void __launch_bounds__(256) panel_nopiv_kernel(T *__restrict__ P, int
     for (int c = 0; c < jb; ++c)
     {
         const T inv = T(1) / D[c][c];
+        const double pa = fabs((double)D[c][c]);
         for (int r = tid; r < nr; r += 256)
+        {
+            if (!(fabs((double)X[r][c]) <= pa) || !(pa > 0.0))
+                atomicOr(&status[3], 1);
             X[r][c] = X[r][c] * inv;
+        }
         __syncthreads();
         for (int idx = tid; idx < NB * NB; idx += 256)
         {
@@ -802,7 +816,7 @@ int ensure_ws()
 
 template <typename T> int panel_rows_max() { return sizeof(T) == 8 ? 256 : 512; }
 
-template <typename T> int factor_impl(int n, T *A, int *piv, int *info_out, bool pivoting)
+template <typename T> int factor_impl(int n, T *A, int *piv, int *info_out, bool pivoting, bool *would_pivot = nullptr)
 {
     if (int st = ensure_ws())
         return st;
@@ -823,6 +837,7 @@ template <typename T> int factor_impl(int n, T *A, int *piv, int *info_out, bool
     // Panel width: 32 up to n = 2048, where single-workgroup register panels cover every
     // panel and the trailing updates are latency-bound anyway; 64 above.
     PVL_HIP(hipMemsetAsync(status + 2, 0x7F, sizeof(int), stream())); // no-pivot zero pivots: min
+    PVL_HIP(hipMemsetAsync(status + 3, 0, sizeof(int), stream()));    // an interchange was due
     const int nbf = !pivoting ? NB : (n <= 2048 ? kPanelWgCols : NB);
     for (int k = 0; k < n; k += nbf)
     {
@@ -887,7 +902,7 @@ template <typename T> int factor_impl(int n, T *A, int *piv, int *info_out, bool
             PVL_HIP(hipGetLastError());
         }
     }
-    int st2[3];
+    int st2[4];
     PVL_HIP(hipMemcpyAsync(st2, status, sizeof(st2), hipMemcpyDeviceToHost, stream()));
     PVL_HIP(hipStreamSynchronize(stream()));
     if (st2[1])
@@ -896,6 +911,8 @@ template <typename T> int factor_impl(int n, T *A, int *piv, int *info_out, bool
         return PVL_ERR_HIP;
     }
     *info_out = pivoting ? st2[0] : (st2[2] == 0x7F7F7F7F ? 0 : st2[2]);
+    if (would_pivot)
+        *would_pivot = !pivoting && st2[3] != 0;
     return PVL_OK;
 }
 
@@ -911,7 +928,7 @@ struct pvl_lu
     int *piv = nullptr;    // device, n
     int *perm = nullptr;   // device, n: row i of P B is row perm[i] of B
     void *work = nullptr;  // n * kMaxRhs (T) staging for permuted right-hand sides
-    bool pivoting = true;  // partial pivoting (LAPACK) or none
+    int pivoting = PVL_PIVOT_PARTIAL;
 };
 
 namespace
@@ -1043,12 +1060,12 @@ int pvl_lu_destroy(pvl_lu *h)
 // enough (refinement reports PVL_ERR_NOTCONV otherwise).
 int pvl_lu_set_pivoting(pvl_lu *h, int pivoting)
 {
-    if (!h)
+    if (!h || (pivoting != PVL_PIVOT_NONE && pivoting != PVL_PIVOT_PARTIAL && pivoting != PVL_PIVOT_AUTO))
     {
-        set_error("null LU handle");
+        set_error("lu_set_pivoting: %s", h ? "unknown mode" : "null LU handle");
         return PVL_ERR_ARG;
     }
-    h->pivoting = pivoting != 0;
+    h->pivoting = pivoting;
     return PVL_OK;
 }
 
@@ -1061,21 +1078,30 @@ int pvl_lu_factor(pvl_lu *h, const double *A_dev, int *info_out)
     const int n = h->n;
     const size_t cnt = (size_t)n * n;
     int info = 0, st;
-    if (h->prec == PVL_FP64)
+    // PVL_PIVOT_AUTO: factor without interchanges; if partial pivoting would have chosen
+    // another row at any column, factor again with partial pivoting.  Either way the
+    // result is partial pivoting's factorisation.
+    const bool try_free = h->pivoting != PVL_PIVOT_PARTIAL;
+    for (int attempt = 0; attempt < 2; ++attempt)
     {
-        PVL_HIP(hipMemcpyAsync(h->lu, A_dev, sizeof(double) * cnt, hipMemcpyDeviceToDevice, stream()));
-        st = factor_impl<double>(n, (double *)h->lu, h->piv, &info, h->pivoting);
-        if (!st)
-            st = finish_factor<double>(h);
+        const bool pivot = attempt == 1 || !try_free;
+        bool would_pivot = false;
+        if (h->prec == PVL_FP64)
+        {
+            PVL_HIP(hipMemcpyAsync(h->lu, A_dev, sizeof(double) * cnt, hipMemcpyDeviceToDevice, stream()));
+            st = factor_impl<double>(n, (double *)h->lu, h->piv, &info, pivot, &would_pivot);
+        }
+        else
+        {
+            f64_to_f32_kernel<<<2048, 256, 0, stream()>>>(A_dev, (float *)h->lu, cnt);
+            PVL_HIP(hipGetLastError());
+            st = factor_impl<float>(n, (float *)h->lu, h->piv, &info, pivot, &would_pivot);
+        }
+        if (st || pivot || !(h->pivoting == PVL_PIVOT_AUTO && would_pivot))
+            break;
     }
-    else
-    {
-        f64_to_f32_kernel<<<2048, 256, 0, stream()>>>(A_dev, (float *)h->lu, cnt);
-        PVL_HIP(hipGetLastError());
-        st = factor_impl<float>(n, (float *)h->lu, h->piv, &info, h->pivoting);
-        if (!st)
-            st = finish_factor<float>(h);
-    }
+    if (!st)
+        st = h->prec == PVL_FP64 ? finish_factor<double>(h) : finish_factor<float>(h);
     if (info_out)
         *info_out = info;
     if (!st && info)

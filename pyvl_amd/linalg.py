@@ -24,22 +24,31 @@ FP64 = _lib.PVL_FP64
 FP32 = _lib.PVL_FP32
 
 
+PIVOT_NONE, PIVOT_PARTIAL, PIVOT_AUTO = 0, 1, 2  # include/pvl.h pvl_pivoting
+
+
 class LU:
     """Device LU factorisation of an n x n matrix (row-major), FP64 or FP32 factors.
 
-    ``pivoting=False`` factors without row interchanges: meant for an FP32 factorisation
-    that FP64 refinement (:func:`solve_refine`) corrects, on diagonally strong systems such
-    as the deflated closed-body ones; the default is LAPACK's partial pivoting."""
+    ``pivoting``: True (default) = LAPACK's partial pivoting; "auto" = the same
+    factorisation, computed without interchanges when partial pivoting would make none
+    (checked at every column) and refactored with partial pivoting otherwise; False = no
+    interchanges at all, meant for an FP32 factorisation that FP64 refinement
+    (:func:`solve_refine`) corrects on diagonally strong systems such as the deflated
+    closed-body ones."""
 
-    def __init__(self, n: int, precision: int = FP64, pivoting: bool = True):
+    def __init__(self, n: int, precision: int = FP64, pivoting=True):
         self.n = int(n)
         self.precision = int(precision)
-        self.pivoting = bool(pivoting)
+        modes = {True: PIVOT_PARTIAL, False: PIVOT_NONE, "auto": PIVOT_AUTO}
+        if pivoting not in modes:
+            raise ValueError(f"pivoting must be True, False or 'auto', got {pivoting!r}")
+        self.pivoting = pivoting
         h = C.c_void_p()
         _lib.call("pvl_lu_create", C.byref(h), C.c_int(self.n), C.c_int(self.precision))
         self._h = h
-        if not self.pivoting:
-            _lib.call("pvl_lu_set_pivoting", h, C.c_int(0))
+        if modes[pivoting] != PIVOT_PARTIAL:
+            _lib.call("pvl_lu_set_pivoting", h, C.c_int(modes[pivoting]))
         self.info = 0
 
     def factor(self, a) -> "LU":

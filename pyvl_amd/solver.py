@@ -165,7 +165,8 @@ class UnsteadySolver:
         self.d_A = DeviceArray((n, n))
         self.d_b = DeviceArray((n,))
         self.d_x = DeviceArray((n,))
-        self.lu = LA.LU(n, LA.FP32 if solve == "ir32" else LA.FP64)
+        # partial pivoting's factorisation; without interchanges when it makes none
+        self.lu = LA.LU(n, LA.FP32 if solve == "ir32" else LA.FP64, pivoting="auto")
         self.vinf = np.ascontiguousarray(flow.vector)
         self.iteration = 0
         self.ir_iters: list[int] = []

@@ -219,3 +219,33 @@ def test_unpivoted_fp32_refinement_closed_body():
     x_ref = sla.lu_solve(sla.lu_factor(ad), b)
     for q in range(b.shape[1]):
         assert _rel(x.numpy()[:, q], x_ref[:, q]) < 1e-10
+
+
+@pytest.mark.parametrize("n,dominant", [(300, True), (1000, True), (2100, True), (300, False), (1000, False), (2100, False)])
+def test_auto_pivoting_is_partial_pivoting(n, dominant):
+    """"auto" gives partial pivoting's factorisation: identity pivots and the unpivoted
+    factors when no interchange is due (diagonally dominant), LAPACK's pivots and factors
+    when one is (random matrix: refactored with partial pivoting)."""
+    rng = np.random.default_rng(70 + n)
+    a = rng.normal(size=(n, n))
+    if dominant:
+        a += 2.0 * n * np.eye(n)
+    lu_ref, piv_ref = sla.lu_factor(a)
+    h = LA.LU(n, LA.FP64, pivoting="auto").factor(a)
+    assert np.array_equal(h.pivots(), piv_ref)
+    assert np.array_equal(piv_ref == np.arange(n), np.full(n, dominant)) or not dominant
+    assert _rel(h.factors(), lu_ref) < 1e-12
+
+
+def test_auto_pivoting_interchange_below_the_diagonal_block():
+    """An interchange due only in rows below the 64 x 64 diagonal block (row 100 beats the
+    diagonal in column 5) is detected: the factorisation is LAPACK's."""
+    n = 256
+    rng = np.random.default_rng(5)
+    a = rng.normal(size=(n, n)) * 0.01 + np.eye(n)
+    a[100, 5] = 3.0
+    lu_ref, piv_ref = sla.lu_factor(a)
+    assert piv_ref[5] == 100
+    h = LA.LU(n, LA.FP64, pivoting="auto").factor(a)
+    assert np.array_equal(h.pivots(), piv_ref)
+    assert _rel(h.factors(), lu_ref) < 1e-12
