@@ -217,6 +217,13 @@ int pvl_wake_info(const pvl_wake *w, unsigned *n_te, unsigned *n_lines, unsigned
  * i.e. get_velocity of :159-175). */
 int pvl_wake_update(pvl_wake *w, double time, const pvl_mesh *bound, const double *bound_pos_dev,
                     const double *circulation_dev, const double *vinf, int mode, int precision);
+/* Scheduling only, no reference counterpart: starts the wake's own induction of the next
+ * pvl_wake_update (mode 1; it needs only the wake's current state) on a second library
+ * stream, so that it runs beside the bound-circulation solve issued in between.  That
+ * pvl_wake_update (same mode and precision, no pvl_wake_set_state in between) then adds the
+ * partial sums in the same order: results are bit-identical to pvl_wake_update alone.  A
+ * no-op for mode 0, an empty wake, or when already started. */
+int pvl_wake_begin_update(pvl_wake *w, int mode, int precision);
 /* apply_corrections (:177-223) on a row-major n_cpts x n_surf matrix and n_cpts rhs. */
 int pvl_wake_apply_corrections(pvl_wake *w, unsigned n_cpts, const double *cpts_dev, const double *normals_dev,
                                double *mat_dev, unsigned n_surf, double *rhs_dev, int precision);

@@ -113,6 +113,7 @@ SIGNATURES: dict[str, tuple] = {
         C.c_int,
         [C.c_void_p, C.c_double, C.c_void_p, _dp, _dp, C.POINTER(C.c_double), C.c_int, C.c_int],
     ),
+    "pvl_wake_begin_update": (C.c_int, [C.c_void_p, C.c_int, C.c_int]),
     "pvl_wake_apply_corrections": (
         C.c_int,
         [C.c_void_p, C.c_uint, _dp, _dp, _dp, C.c_uint, _dp, C.c_int],

@@ -39,6 +39,7 @@ __global__ void __launch_bounds__(256) line_geom_kernel(unsigned n_lines, const 
                                                        const double *__restrict__ pos, double tol,
                                                        LineGeom *__restrict__ out)
 {
+    main_stream_priority();
     const unsigned i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n_lines)
         return;
@@ -75,6 +76,7 @@ __global__ void __launch_bounds__(kBlock)
                              const double *__restrict__ cpts, const double *__restrict__ normals,
                              double *__restrict__ out, double tol)
 {
+    main_stream_priority();
     constexpr int kSurfTile = 4 * SURF_PER_WAVE;
     constexpr int kComp = NORMAL ? 1 : 3;
     constexpr int kRow = kSurfTile * kComp + 1; // padded LDS row (doubles)
@@ -182,6 +184,7 @@ __global__ void __launch_bounds__(kBlock)
     line_induction_kernel(unsigned n_cpts, unsigned n_lines, const LineGeom *__restrict__ geom,
                           const double *__restrict__ cpts, double *__restrict__ out, double tol)
 {
+    main_stream_priority();
     constexpr int kLineTile = 4 * LINES_PER_WAVE;
     constexpr int kRow = kLineTile * 3 + 1;
     __shared__ double tile[kCptTile * kRow];
@@ -225,6 +228,7 @@ __global__ void __launch_bounds__(256) line_circulation_kernel(unsigned n_lines,
                                                                const double *__restrict__ circ,
                                                                double *__restrict__ gamma)
 {
+    main_stream_priority();
     const unsigned i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n_lines)
         return;
@@ -240,46 +244,58 @@ __global__ void __launch_bounds__(256) line_circulation_kernel(unsigned n_lines,
 
 // One lane per point; the workgroup's four waves split a chunk of lines, partial sums
 // are combined in LDS in wave order and written per chunk; a second pass adds the chunks
-// in chunk order.  No atomics: the result does not depend on scheduling.
+// in chunk order.  No atomics: the result does not depend on scheduling.  Work items are
+// (64-point block, line chunk) pairs, item = chunk * pblocks + block; workgroup b takes items
+// b, b + gridDim.x, .. (a grid smaller than the item count leaves CU slots to other work on
+// the device; each item's arithmetic is the same either way).
 // SITE only gives the free-wake self-induction launch its own symbol for profilers.
+// At most 80 VGPRs (6 waves per SIMD): with 4 workgroups per CU on the side stream, each
+// SIMD keeps 192 VGPRs for the library stream's kernels.
 template <int PREC, int SITE>
-__global__ void __launch_bounds__(kBlock)
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6)))
     point_velocity_kernel(unsigned n_pts, const double *__restrict__ pts, unsigned n_lines,
                           const LineGeom *__restrict__ geom, const double *__restrict__ gamma,
-                          unsigned lines_per_chunk, double *__restrict__ partial, double tol)
+                          unsigned lines_per_chunk, double *__restrict__ partial, double tol, unsigned pblocks,
+                          unsigned n_items)
 {
+    if constexpr (SITE == 0)
+        main_stream_priority();
     __shared__ double red[4][3][kWave];
     const unsigned w = wave_id(), lane = lane_id();
-    const unsigned p = blockIdx.x * kWave + lane;
-    const unsigned pc = p < n_pts ? p : n_pts - 1;
-    const double px = pts[3 * pc + 0], py = pts[3 * pc + 1], pz = pts[3 * pc + 2];
     const double tol2 = tol * tol;
-    const unsigned chunk = blockIdx.y;
-    const unsigned l0 = chunk * lines_per_chunk;
-    const unsigned l1 = (l0 + lines_per_chunk) < n_lines ? (l0 + lines_per_chunk) : n_lines;
     const unsigned per_wave = (lines_per_chunk + 3) / 4;
-    const unsigned a = l0 + w * per_wave;
-    const unsigned b = (a + per_wave) < l1 ? (a + per_wave) : l1;
-    double vx = 0.0, vy = 0.0, vz = 0.0;
-    for (unsigned l = a; l < b; ++l)
-        line_k_acc<PREC>(geom[l], px, py, pz, tol, tol2, gamma[l], vx, vy, vz);
-    red[w][0][lane] = vx;
-    red[w][1][lane] = vy;
-    red[w][2][lane] = vz;
-    __syncthreads();
-    if (w == 0 && p < n_pts)
+    for (unsigned item = blockIdx.x; item < n_items; item += gridDim.x)
     {
-        double sx = red[0][0][lane], sy = red[0][1][lane], sz = red[0][2][lane];
-        for (int q = 1; q < 4; ++q)
+        const unsigned chunk = item / pblocks;
+        const unsigned p = (item - chunk * pblocks) * kWave + lane;
+        const unsigned pc = p < n_pts ? p : n_pts - 1;
+        const double px = pts[3 * pc + 0], py = pts[3 * pc + 1], pz = pts[3 * pc + 2];
+        const unsigned l0 = chunk * lines_per_chunk;
+        const unsigned l1 = (l0 + lines_per_chunk) < n_lines ? (l0 + lines_per_chunk) : n_lines;
+        const unsigned a = l0 + w * per_wave;
+        const unsigned b = (a + per_wave) < l1 ? (a + per_wave) : l1;
+        double vx = 0.0, vy = 0.0, vz = 0.0;
+        for (unsigned l = a; l < b; ++l)
+            line_k_acc<PREC>(geom[l], px, py, pz, tol, tol2, gamma[l], vx, vy, vz);
+        red[w][0][lane] = vx;
+        red[w][1][lane] = vy;
+        red[w][2][lane] = vz;
+        __syncthreads();
+        if (w == 0 && p < n_pts)
         {
-            sx = sx + red[q][0][lane];
-            sy = sy + red[q][1][lane];
-            sz = sz + red[q][2][lane];
+            double sx = red[0][0][lane], sy = red[0][1][lane], sz = red[0][2][lane];
+            for (int q = 1; q < 4; ++q)
+            {
+                sx = sx + red[q][0][lane];
+                sy = sy + red[q][1][lane];
+                sz = sz + red[q][2][lane];
+            }
+            double *dst = partial + ((size_t)chunk * n_pts + p) * 3;
+            dst[0] = sx;
+            dst[1] = sy;
+            dst[2] = sz;
         }
-        double *dst = partial + ((size_t)chunk * n_pts + p) * 3;
-        dst[0] = sx;
-        dst[1] = sy;
-        dst[2] = sz;
+        __syncthreads(); // red is rewritten by the next item
     }
 }
 
@@ -287,6 +303,7 @@ __global__ void __launch_bounds__(256) chunk_reduce_kernel(unsigned n_pts, unsig
                                                            const double *__restrict__ partial,
                                                            double *__restrict__ out, int accumulate)
 {
+    main_stream_priority();
     const unsigned i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= 3 * n_pts)
         return;
@@ -305,6 +322,7 @@ __global__ void __launch_bounds__(256) surface_geometry_kernel(unsigned n_surf, 
                                                                double *__restrict__ normal_out,
                                                                double *__restrict__ center_out)
 {
+    main_stream_priority();
     const unsigned s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= n_surf)
         return;
@@ -445,6 +463,69 @@ namespace pvl
 // Point-velocity sum over the first n_active lines of the mesh (all lines for the public
 // entry point; the wake passes the prefix of lines its filled rows use -- the other lines
 // carry zero circulation).
+// Chunking: aim for >= ~4 workgroups per CU while keeping >= 64 lines per wave.
+static void plan_chunks(unsigned n_pts, unsigned n_active, unsigned *n_chunks_out, unsigned *lpc_out)
+{
+    const unsigned pblocks = (n_pts + kWave - 1) / kWave;
+    unsigned n_chunks = (2048 + pblocks - 1) / pblocks;
+    const unsigned max_chunks = (n_active + 255) / 256;
+    if (n_chunks > max_chunks)
+        n_chunks = max_chunks;
+    if (n_chunks < 1)
+        n_chunks = 1;
+    if (n_chunks > 65535u)
+        n_chunks = 65535u;
+    const unsigned lpc = (n_active + n_chunks - 1) / n_chunks;
+    *n_chunks_out = (n_active + lpc - 1) / lpc;
+    *lpc_out = lpc;
+}
+
+unsigned point_velocity_chunks(unsigned n_pts, unsigned n_active)
+{
+    unsigned nc, lpc;
+    plan_chunks(n_pts, n_active, &nc, &lpc);
+    return nc;
+}
+
+size_t line_geom_bytes(unsigned n_lines) { return sizeof(LineGeom) * (size_t)(n_lines ? n_lines : 1); }
+
+int point_velocity_partials(const pvl_mesh *m, double tol, const double *pos, const double *circ, unsigned n_pts,
+                            const double *pts, int precision, unsigned n_active, const char *prof_name,
+                            hipStream_t s, void *geom, double *gamma, double *partial, unsigned max_groups)
+{
+    LineGeom *g = (LineGeom *)geom;
+    line_geom_kernel<<<(n_active + 255) / 256, 256, 0, s>>>(n_active, m->d_lines, pos, tol, g);
+    PVL_HIP(hipGetLastError());
+    line_circulation_kernel<<<(n_active + 255) / 256, 256, 0, s>>>(n_active, m->d_line_offsets, m->d_line_surfs,
+                                                                     circ, gamma);
+    PVL_HIP(hipGetLastError());
+    unsigned n_chunks, lpc;
+    plan_chunks(n_pts, n_active, &n_chunks, &lpc);
+    const unsigned pblocks = (n_pts + kWave - 1) / kWave;
+    const unsigned n_items = pblocks * n_chunks;
+    const unsigned grid = max_groups && max_groups < n_items ? max_groups : n_items;
+    ProfScope prof(prof_name, (double)n_pts * (double)n_active, s);
+    const bool self_site = std::strcmp(prof_name, "wake_self_velocity") == 0;
+    dispatch_precision(precision, [&](auto P) {
+        if (self_site)
+            point_velocity_kernel<decltype(P)::value, 1>
+                <<<grid, kBlock, 0, s>>>(n_pts, pts, n_active, g, gamma, lpc, partial, tol, pblocks, n_items);
+        else
+            point_velocity_kernel<decltype(P)::value, 0>
+                <<<grid, kBlock, 0, s>>>(n_pts, pts, n_active, g, gamma, lpc, partial, tol, pblocks, n_items);
+    });
+    PVL_HIP(hipGetLastError());
+    return PVL_OK;
+}
+
+int point_velocity_reduce(unsigned n_pts, unsigned n_chunks, const double *partial, double *out, int accumulate,
+                          hipStream_t s)
+{
+    chunk_reduce_kernel<<<(3 * n_pts + 255) / 256, 256, 0, s>>>(n_pts, n_chunks, partial, out, accumulate);
+    PVL_HIP(hipGetLastError());
+    return PVL_OK;
+}
+
 int point_velocity_impl(const pvl_mesh *m, double tol, const double *pos, const double *circ, unsigned n_pts,
                         const double *pts, double *out, int accumulate, int precision, unsigned n_active,
                         const char *prof_name)
@@ -459,47 +540,16 @@ int point_velocity_impl(const pvl_mesh *m, double tol, const double *pos, const 
             PVL_HIP(hipMemsetAsync(out, 0, sizeof(double) * 3 * n_pts, stream()));
         return PVL_OK;
     }
-    LineGeom *g;
-    if (int st = compute_line_geom(m, pos, tol, &g, n_active, 0))
-        return st;
+    const unsigned n_chunks = point_velocity_chunks(n_pts, n_active);
+    void *g = scratch(0, line_geom_bytes(n_active));
     double *gamma = (double *)scratch(1, sizeof(double) * n_active);
-    if (!gamma)
-        return PVL_ERR_NOMEM;
-    line_circulation_kernel<<<(n_active + 255) / 256, 256, 0, stream()>>>(n_active, m->d_line_offsets,
-                                                                            m->d_line_surfs, circ, gamma);
-    PVL_HIP(hipGetLastError());
-    // Chunking: aim for >= ~4 workgroups per CU while keeping >= 64 lines per wave.
-    const unsigned pblocks = (n_pts + kWave - 1) / kWave;
-    unsigned n_chunks = (2048 + pblocks - 1) / pblocks;
-    const unsigned max_chunks = (n_active + 255) / 256;
-    if (n_chunks > max_chunks)
-        n_chunks = max_chunks;
-    if (n_chunks < 1)
-        n_chunks = 1;
-    if (n_chunks > 65535u)
-        n_chunks = 65535u;
-    const unsigned lpc = (n_active + n_chunks - 1) / n_chunks;
-    n_chunks = (n_active + lpc - 1) / lpc;
     double *partial = (double *)scratch(2, sizeof(double) * 3 * (size_t)n_pts * n_chunks);
-    if (!partial)
+    if (!g || !gamma || !partial)
         return PVL_ERR_NOMEM;
-    const dim3 grid(pblocks, n_chunks);
-    {
-        ProfScope prof(prof_name, (double)n_pts * (double)n_active);
-        const bool self_site = std::strcmp(prof_name, "wake_self_velocity") == 0;
-        dispatch_precision(precision, [&](auto P) {
-            if (self_site)
-                point_velocity_kernel<decltype(P)::value, 1>
-                    <<<grid, kBlock, 0, stream()>>>(n_pts, pts, n_active, g, gamma, lpc, partial, tol);
-            else
-                point_velocity_kernel<decltype(P)::value, 0>
-                    <<<grid, kBlock, 0, stream()>>>(n_pts, pts, n_active, g, gamma, lpc, partial, tol);
-        });
-        PVL_HIP(hipGetLastError());
-    }
-    chunk_reduce_kernel<<<(3 * n_pts + 255) / 256, 256, 0, stream()>>>(n_pts, n_chunks, partial, out, accumulate);
-    PVL_HIP(hipGetLastError());
-    return PVL_OK;
+    if (int st = point_velocity_partials(m, tol, pos, circ, n_pts, pts, precision, n_active, prof_name, stream(), g,
+                                         gamma, partial, 0))
+        return st;
+    return point_velocity_reduce(n_pts, n_chunks, partial, out, accumulate, stream());
 }
 
 } // namespace pvl

@@ -11,7 +11,7 @@
 //                  grid-wide exchange per column carries every group's pivot candidate
 //                  together with the candidate row, so a column costs one barrier.
 //   laswp_kernel   applies the panel's interchanges to the other columns.
-//   trsm_kernel    U12 = L11^-1 A12 (one thread per column, L11 in LDS).
+//   trsm_kernel    U12 = L11^-1 A12 (one lane per column in registers, L11 in LDS).
 //   gemm_kernel    A22 -= L21 U12 on MFMA (v_mfma_f64_16x16x4f64 / v_mfma_f32_16x16x4f32).
 // Solves run one launch per 64-row block (update of the later blocks, then a wave-level
 // substitution with the next diagonal block).
@@ -22,6 +22,7 @@
 
 #include <climits>
 #include <cmath>
+#include <utility>
 #include <vector>
 
 using namespace pvl;
@@ -135,6 +136,7 @@ __global__ void __launch_bounds__(256) panel_kernel(T *__restrict__ P, int lda, 
                                                     int *__restrict__ piv, PanelX<T> *__restrict__ X, unsigned base,
                                                     int *__restrict__ status)
 {
+    main_stream_priority<2>();
     extern __shared__ __align__(16) unsigned char smem_raw[];
     constexpr int LDP = NB + 1;
     T *L = (T *)smem_raw;
@@ -288,98 +290,233 @@ __global__ void __launch_bounds__(256) panel_kernel(T *__restrict__ P, int lda, 
     }
 }
 
-// Panel factorisation WITHOUT pivoting (for an FP32 factorisation ahead of FP64 refinement on
-// diagonally strong systems): every workgroup factors the jb x jb diagonal block of the
-// panel in LDS (column sweep, the same arithmetic everywhere, so the copies agree);
-// workgroup 0 stores L11 / U11 and the identity interchanges, workgroup g >= 1 solves its
-// 64 rows of the panel below the diagonal block, X U11 = A21 (column sweep), and stores
-// L21.  No workgroup waits for another.  status[0]: the first exactly-zero pivot.
-template <typename T>
-__global__ void __launch_bounds__(256) panel_nopiv_kernel(T *__restrict__ P, int lda, int m, int jb, int row_base,
-                                                          int *__restrict__ piv, int *__restrict__ status)
+__device__ __forceinline__ double bcast_lane(double v, int k)
 {
-    __shared__ T D[NB][NB + 1];
-    __shared__ T X[NB][NB + 1];
-    const int tid = threadIdx.x, g = blockIdx.x;
-    for (int idx = tid; idx < NB * NB; idx += 256)
+    const unsigned long long u = (unsigned long long)__double_as_longlong(v);
+    const unsigned lo = __builtin_amdgcn_readlane((unsigned)u, k);
+    const unsigned hi = __builtin_amdgcn_readlane((unsigned)(u >> 32), k);
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+__device__ __forceinline__ float bcast_lane(float v, int k)
+{
+    return __uint_as_float(__builtin_amdgcn_readlane(__float_as_uint(v), k));
+}
+
+// Value of lane 4 (lane / 4) + P in every lane of each quad (DPP quad_perm).
+template <int P> __device__ __forceinline__ double quad_bcast(double v)
+{
+    constexpr int ctrl = P | (P << 2) | (P << 4) | (P << 6);
+    const unsigned long long u = (unsigned long long)__double_as_longlong(v);
+    const unsigned lo = (unsigned)__builtin_amdgcn_mov_dpp((int)(unsigned)u, ctrl, 0xF, 0xF, false);
+    const unsigned hi = (unsigned)__builtin_amdgcn_mov_dpp((int)(unsigned)(u >> 32), ctrl, 0xF, 0xF, false);
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+template <int P> __device__ __forceinline__ float quad_bcast(float v)
+{
+    constexpr int ctrl = P | (P << 2) | (P << 4) | (P << 6);
+    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), ctrl, 0xF, 0xF, false));
+}
+
+// Panel step WITHOUT interchanges (pivoting modes "none" and "auto"), fused: the LU of the
+// 64 x 64 diagonal block, L21 = A21 U11^-1 and U12 = L11^-1 A12 in one launch.  A workgroup
+// has 6 waves, each holding 16 rows or columns with 4 lanes per row / column (lane p of a quad
+// holds entries 4k + p, k = 0..15, in registers):
+//   waves 0-3  its own copy of the diagonal block (wave w: rows 16w .. 16w + 15),
+//   wave 4     panel rows jb + 16g .. jb + 16g + 15 below the diagonal block (L21),
+//   wave 5     columns jb + 16g .. jb + 16g + 15 of the panel's row block right of it (U12).
+// Step j: the quad owning diagonal row j has published it and 1/u_jj in LDS (as soon as step
+// j - 1 finalised it); after one barrier every later diagonal row and every row below forms
+// its multiplier (owner lane of column j), broadcasts it over the quad (DPP) and subtracts it
+// times row j, and the diagonal rows publish their multipliers l_ij, which the U12 columns
+// apply one step later (x_i -= l_i,j-1 x_j-1).  The diagonal waves carry the step-to-step
+// chain and run at the highest priority; the other two waves fill in behind them.  Per element the
+// operations and their order are those of the right-looking column sweep (dgetf2 without
+// interchanges, dtrsm).  The copies of the diagonal block agree (same arithmetic
+// everywhere); workgroup 0 stores L11 \ U11 and the identity interchanges.  Only the last
+// panel can be narrower than 64 (jb < 64 implies m == jb: no rows below, no columns right):
+// it is padded with identity rows and columns.
+// status[2]: smallest zero-pivot column (atomicMin); status[3] |= 1 when partial pivoting
+// would interchange: a sub-diagonal |a| above |pivot| at its column's elimination, or a zero
+// or NaN pivot.
+constexpr int kNopivRows = 16;   // rows below (and columns right) per workgroup
+constexpr int kNopivThreads = 384;
+
+// Registers and LDS rings of panel_nopiv_kernel (one instance per lane).
+template <typename T> struct NopivLane
+{
+    static constexpr int KQ = NB / 4;
+    T v[KQ];       // diagonal row, row below, or column right (by role)
+    T (*Urow)[NB]; // row j of U11, column 4k + p at p * KQ + k
+    T (*Lcol)[NB]; // column j of L11, row 4k + p at p * KQ + k
+    T *Uinv;
+    int role;      // 0 diagonal block, 1 row below, 2 column right
+    int p, rd, jb, row_base, first_zero;
+    bool live, due;
+};
+
+// Publish diagonal row J (final once step J - 1 is done) and 1/u_JJ into ring slot J & 1.
+template <int J, typename T> __device__ __forceinline__ void nopiv_publish(NopivLane<T> &L)
+{
+    constexpr int KQ = NopivLane<T>::KQ;
+    constexpr int kj = J >> 2, pj = J & 3, s = J & 1;
+    if (L.rd == J)
     {
-        const int r = idx / NB, c = idx % NB;
-        D[r][c] = (r < jb && c < jb) ? P[(size_t)r * lda + c] : (r == c ? T(1) : T(0));
+#pragma unroll
+        for (int k = 0; k < KQ; ++k)
+            L.Urow[s][L.p * KQ + k] = L.v[k];
+        if (L.p == pj)
+            L.Uinv[s] = T(1) / L.v[kj];
     }
+}
+
+// Step J of panel_nopiv_kernel (J a compile-time constant: every register index is static).
+template <int J, typename T> __device__ __forceinline__ void nopiv_step(NopivLane<T> &L)
+{
+    constexpr int KQ = NopivLane<T>::KQ;
+    constexpr int kj = J >> 2, pj = J & 3, s = J & 1;
+    const int p = L.p;
     __syncthreads();
-    // D = L11 U11 (right-looking, no interchanges).  status[3] records whether partial
-    // pivoting would have interchanged: some |a_rj| > |a_jj| (or a zero / NaN pivot) at the
-    // column's elimination.
-    for (int j = 0; j < jb - 1; ++j)
+    if (L.role != 2)
     {
-        if (g == 0)
+        const T ujj = L.Urow[s][pj * KQ + kj];
+        const T inv = L.Uinv[s];
+        const double pa = fabs((double)ujj);
+        T u[KQ];
+#pragma unroll
+        for (int k = kj; k < KQ; ++k)
+            u[k] = L.Urow[s][p * KQ + k];
+        if (L.role == 0)
         {
-            const double pa = fabs((double)D[j][j]);
-            for (int r = j + 1 + tid; r < jb; r += 256)
-                if (!(fabs((double)D[r][j]) <= pa) || !(pa > 0.0))
-                    atomicOr(&status[3], 1);
+            if (J < L.jb && ujj == T(0) && L.first_zero == 0)
+                L.first_zero = L.row_base + J + 1;
+            if (L.rd > J)
+            {
+                const T own = L.v[kj];
+                T l = own * inv;
+                if (p == pj)
+                {
+                    L.due |= J < L.jb && (!(fabs((double)own) <= pa) || !(pa > 0.0));
+                    L.v[kj] = l;
+                    L.Lcol[s][(L.rd & 3) * KQ + (L.rd >> 2)] = l;
+                }
+                l = quad_bcast<pj>(l);
+#pragma unroll
+                for (int k = kj; k < KQ; ++k)
+                {
+                    const T v = fma(-l, u[k], L.v[k]);
+                    L.v[k] = (k > kj || p > pj) ? v : L.v[k];
+                }
+            }
+            if constexpr (J + 1 < NB)
+                nopiv_publish<J + 1>(L);
         }
-        const T inv = T(1) / D[j][j];
-        for (int idx = tid; idx < NB * NB; idx += 256)
+        else
         {
-            const int r = idx / NB, c = idx % NB;
-            if (r > j && r < jb && c > j && c < jb)
-                D[r][c] = fma(-(D[r][j] * inv), D[j][c], D[r][c]);
+            const T own = L.v[kj];
+            T l = own * inv;
+            if (p == pj)
+            {
+                L.due |= L.live && (!(fabs((double)own) <= pa) || !(pa > 0.0));
+                L.v[kj] = l;
+            }
+            l = quad_bcast<pj>(l);
+#pragma unroll
+            for (int k = kj; k < KQ; ++k)
+            {
+                const T v = fma(-l, u[k], L.v[k]);
+                L.v[k] = (k > kj || p > pj) ? v : L.v[k];
+            }
         }
-        __syncthreads();
-        for (int r = j + 1 + tid; r < jb; r += 256)
-            D[r][j] = D[r][j] * inv;
-        __syncthreads();
     }
-    if (g == 0)
+    else if constexpr (J >= 1)
     {
-        for (int idx = tid; idx < jb * jb; idx += 256)
+        // U12 columns: apply column J - 1 of L11 (published last step)
+        constexpr int t = J - 1, kt = t >> 2, pt = t & 3;
+        const T yt = quad_bcast<pt>(L.v[kt]);
+#pragma unroll
+        for (int k = kt; k < KQ; ++k)
         {
-            const int r = idx / jb, c = idx % jb;
-            P[(size_t)r * lda + c] = D[r][c];
+            const T v = fma(-L.Lcol[t & 1][p * KQ + k], yt, L.v[k]);
+            L.v[k] = (k > kt || p > pt) ? v : L.v[k];
         }
-        for (int j = tid; j < jb; j += 256)
-        {
-            piv[row_base + j] = row_base + j;
-            if (D[j][j] == T(0))
-                atomicMin(&status[2], row_base + j + 1); // smallest zero-pivot column
-        }
-        return;
     }
-    // rows r0 .. r0 + 63 of the panel below the diagonal block
-    const int r0 = jb + (g - 1) * NB;
-    const int nr = min(NB, m - r0);
-    for (int idx = tid; idx < NB * NB; idx += 256)
+    // Keep this step's updates here: the rows below and U12 columns are otherwise only read
+    // by the final stores, and the compiler would sink their FMAs past the barriers.
+#pragma unroll
+    for (int k = 0; k < KQ; ++k)
+        asm volatile("" : "+v"(L.v[k]));
+}
+
+template <typename T, int... J> __device__ __forceinline__ void nopiv_steps(NopivLane<T> &L, std::integer_sequence<int, J...>)
+{
+    (nopiv_step<J>(L), ...);
+}
+
+// At most 96 VGPRs (5 waves per SIMD): two of its waves then fit on a SIMD beside four waves
+// of the side stream's point-velocity kernel.
+template <typename T>
+__global__ void __launch_bounds__(kNopivThreads) __attribute__((amdgpu_waves_per_eu(5))) panel_nopiv_kernel(T *__restrict__ P, int lda, int m, int jb,
+                                                                    int row_base, int *__restrict__ piv,
+                                                                    int *__restrict__ status)
+{
+    constexpr int KQ = NB / 4;
+    __shared__ T Urow[2][NB];
+    __shared__ T Lcol[2][NB];
+    __shared__ T Uinv[2];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int g = blockIdx.x;
+    NopivLane<T> L;
+    L.Urow = Urow, L.Lcol = Lcol, L.Uinv = Uinv;
+    L.role = w < 4 ? 0 : w - 3;
+    L.p = lane & 3;
+    L.rd = L.role == 0 ? 16 * w + (lane >> 2) : -1; // diagonal-block row of this quad
+    L.jb = jb, L.row_base = row_base, L.first_zero = 0, L.due = false;
+    const int rc = jb + kNopivRows * g + (lane >> 2); // panel row below / column right
+    L.live = L.role != 0 && rc < m;                   // (m - jb rows below, m - jb columns right)
+    const int p = L.p, rd = L.rd;
+    if (L.role == 0)
+        main_stream_priority<3>();
+    else
+        main_stream_priority<2>();
+#pragma unroll
+    for (int k = 0; k < KQ; ++k)
     {
-        const int r = idx / NB, c = idx % NB;
-        X[r][c] = (r < nr && c < jb) ? P[(size_t)(r0 + r) * lda + c] : T(0);
+        const int c = 4 * k + p;
+        if (L.role == 0)
+            L.v[k] = (rd < jb && c < jb) ? P[(size_t)rd * lda + c] : (rd == c ? T(1) : T(0));
+        else if (L.role == 1)
+            L.v[k] = L.live ? P[(size_t)rc * lda + c] : T(0);
+        else
+            L.v[k] = L.live ? P[(size_t)c * lda + rc] : T(0);
     }
-    __syncthreads();
-    // X U11 = A21: column c final once the earlier columns are folded in
-    for (int c = 0; c < jb; ++c)
+    nopiv_publish<0>(L);
+    nopiv_steps(L, std::make_integer_sequence<int, NB>{});
+#pragma unroll
+    for (int k = 0; k < KQ; ++k)
     {
-        const T inv = T(1) / D[c][c];
-        const double pa = fabs((double)D[c][c]);
-        for (int r = tid; r < nr; r += 256)
+        const int c = 4 * k + p;
+        if (L.role == 0)
         {
-            if (!(fabs((double)X[r][c]) <= pa) || !(pa > 0.0))
-                atomicOr(&status[3], 1);
-            X[r][c] = X[r][c] * inv;
+            if (g == 0 && rd < jb && c < jb)
+                P[(size_t)rd * lda + c] = L.v[k];
         }
-        __syncthreads();
-        for (int idx = tid; idx < NB * NB; idx += 256)
+        else if (L.live)
         {
-            const int r = idx / NB, cc = idx % NB;
-            if (r < nr && cc > c && cc < jb)
-                X[r][cc] = fma(-X[r][c], D[c][cc], X[r][cc]);
+            if (L.role == 1)
+                P[(size_t)rc * lda + c] = L.v[k];
+            else
+                P[(size_t)c * lda + rc] = L.v[k];
         }
-        __syncthreads();
     }
-    for (int idx = tid; idx < nr * jb; idx += 256)
+    if (g == 0 && L.role == 0)
     {
-        const int r = idx / jb, c = idx % jb;
-        P[(size_t)(r0 + r) * lda + c] = X[r][c];
+        if (rd < jb && p == 0)
+            piv[row_base + rd] = row_base + rd;
+        if (threadIdx.x == 0 && L.first_zero)
+            atomicMin(&status[2], L.first_zero);
     }
+    if (__ballot(L.due) && lane == 0)
+        atomicOr(&status[3], 1);
 }
 
 // Row interchanges of one panel applied to every column outside it.
@@ -387,6 +524,7 @@ template <typename T>
 __global__ void __launch_bounds__(256) laswp_kernel(T *__restrict__ A, int lda, int n, int k, int jb,
                                                    const int *__restrict__ piv)
 {
+    main_stream_priority();
     const int c = blockIdx.x * 256 + threadIdx.x;
     if (c >= n || (c >= k && c < k + jb))
         return;
@@ -402,36 +540,60 @@ __global__ void __launch_bounds__(256) laswp_kernel(T *__restrict__ A, int lda, 
     }
 }
 
-// U12 = L11^-1 A12 for the columns right of the panel: one workgroup per 64-column tile,
-// the tile and unit-lower L11 in LDS, right-looking sweep (step t finalises row t and
-// updates rows t+1.. of all 64 columns; 256 threads = 64 columns x 4 row groups).
-template <typename T>
-__global__ void __launch_bounds__(256) trsm_kernel(T *__restrict__ A, int lda, int n, int k, int jb)
+// U12 = L11^-1 A12 for the columns right of the panel.  Each wave covers 16 columns with 4
+// lanes per column: lane p of a quad holds rows p, p + 4, .. of its column in registers.  Step
+// t broadcasts the final x_t from its owner lane over the quad (DPP) and every lane subtracts
+// l_it x_t from its rows i > t, with the l_it each lane needs contiguous in LDS
+// (Lq[t][p * KQ + k] = l_(4k+p),t).  Per element the operations are those of dtrsm's column
+// sweep.  No barrier after the load.  JB = panel width (32 or 64).
+template <typename T, int JB>
+__global__ void __launch_bounds__(256) trsm_kernel(T *__restrict__ A, int lda, int n, int k)
 {
-    __shared__ T Ls[NB][NB + 1];
-    __shared__ T Xs[NB][NB + 1];
-    const int tid = threadIdx.x;
-    const int c0 = k + jb + blockIdx.x * NB;
-    for (int idx = tid; idx < NB * NB; idx += 256)
+    main_stream_priority();
+    constexpr int KQ = JB / 4;
+    __shared__ T Lq[JB][JB];
+    for (int idx = threadIdx.x; idx < JB * JB; idx += 256)
     {
-        const int r = idx / NB, c = idx % NB;
-        Ls[r][c] = (r < jb && c < jb && c < r) ? A[(size_t)(k + r) * lda + k + c] : T(0);
-        Xs[r][c] = (r < jb && c0 + c < n) ? A[(size_t)(k + r) * lda + c0 + c] : T(0);
+        const int i = idx / JB, t = idx % JB;
+        Lq[t][(i & 3) * KQ + (i >> 2)] = t < i ? A[(size_t)(k + i) * lda + k + t] : T(0);
     }
     __syncthreads();
-    const int c = tid & 63, rg = tid >> 6;
-    for (int t = 0; t < jb - 1; ++t)
+    const int lane = threadIdx.x & 63, p = lane & 3;
+    const int col = k + JB + (blockIdx.x * 4 + (threadIdx.x >> 6)) * 16 + (lane >> 2);
+    if (k + JB + (blockIdx.x * 4 + (threadIdx.x >> 6)) * 16 >= n)
+        return;
+    const bool live = col < n;
+    T x[KQ];
+#pragma unroll
+    for (int q = 0; q < KQ; ++q)
+        x[q] = live ? A[(size_t)(k + 4 * q + p) * lda + col] : T(0);
+#pragma unroll
+    for (int t = 0; t < JB - 1; ++t)
     {
-        const T xt = Xs[t][c];
-        for (int i = t + 1 + rg; i < jb; i += 4)
-            Xs[i][c] = fma(-Ls[i][t], xt, Xs[i][c]);
-        __syncthreads();
+        const int kt = t >> 2, pt = t & 3;
+        T xt;
+        switch (pt)
+        {
+        case 0: xt = quad_bcast<0>(x[kt]); break;
+        case 1: xt = quad_bcast<1>(x[kt]); break;
+        case 2: xt = quad_bcast<2>(x[kt]); break;
+        default: xt = quad_bcast<3>(x[kt]); break;
+        }
+        const T *lrow = &Lq[t][p * KQ];
+        {
+            const T v = fma(-lrow[kt], xt, x[kt]);
+            if (p > pt)
+                x[kt] = v;
+        }
+#pragma unroll
+        for (int q = kt + 1; q < KQ; ++q)
+            x[q] = fma(-lrow[q], xt, x[q]);
     }
-    for (int idx = tid; idx < NB * NB; idx += 256)
+    if (live)
     {
-        const int r = idx / NB, cc = idx % NB;
-        if (r < jb && c0 + cc < n)
-            A[(size_t)(k + r) * lda + c0 + cc] = Xs[r][cc];
+#pragma unroll
+        for (int q = 0; q < KQ; ++q)
+            A[(size_t)(k + 4 * q + p) * lda + col] = x[q];
     }
 }
 
@@ -441,6 +603,7 @@ __global__ void __launch_bounds__(256) trsm_kernel(T *__restrict__ A, int lda, i
 template <typename T>
 __global__ void __launch_bounds__(256) gemm_kernel(T *__restrict__ A, int lda, int m2, int n2, int k, int jb)
 {
+    main_stream_priority<2>();
     constexpr int BM = 64, BN = 64;
     __shared__ T As[BM][NB + 1];
     __shared__ T Bs[NB][BN + 1];
@@ -506,6 +669,7 @@ template <typename T>
 __global__ void __launch_bounds__(256) permute_rows_kernel(const T *__restrict__ B, T *__restrict__ Y, int n, int nrhs,
                                                            const int *__restrict__ perm)
 {
+    main_stream_priority();
     const int idx = blockIdx.x * 256 + threadIdx.x;
     if (idx >= n * nrhs)
         return;
@@ -514,17 +678,6 @@ __global__ void __launch_bounds__(256) permute_rows_kernel(const T *__restrict__
 }
 
 // Value of lane k (wave-uniform k) in every lane, through readlane (no LDS round trip).
-__device__ __forceinline__ double bcast_lane(double v, int k)
-{
-    const unsigned long long u = (unsigned long long)__double_as_longlong(v);
-    const unsigned lo = __builtin_amdgcn_readlane((unsigned)u, k);
-    const unsigned hi = __builtin_amdgcn_readlane((unsigned)(u >> 32), k);
-    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
-}
-__device__ __forceinline__ float bcast_lane(float v, int k)
-{
-    return __uint_as_float(__builtin_amdgcn_readlane(__float_as_uint(v), k));
-}
 
 // Block triangular solve on B (n x nrhs, row-major), in place, one launch per 64-row block
 // in solve order.  LOWER: unit lower L, forward order; else upper U, backward order.
@@ -537,6 +690,7 @@ template <typename T, bool LOWER>
 __global__ void __launch_bounds__(256) trsv_step_kernel(const T *__restrict__ A, int lda, int n, T *__restrict__ B,
                                                         int nrhs, int step)
 {
+    main_stream_priority<3>();
     __shared__ T Ls[NB][NB + 1];
     __shared__ T xs[NB][kMaxRhs + 1];
     __shared__ T acc[NB][kMaxRhs + 1];
@@ -632,21 +786,24 @@ __global__ void __launch_bounds__(256) trsv_step_kernel(const T *__restrict__ A,
     }
 }
 
-// Whole triangular solve in ONE workgroup (1024 threads) for small systems: x (n x nrhs)
-// lives in LDS; blocks of 64 rows are taken in solve order, each first reduced by the
-// already solved part (dot products over contiguous factor rows, 4 rows per wave) and then
+// Whole triangular solve in ONE workgroup (512 threads, at most 96 VGPRs: it fits on a CU
+// beside the side stream's point-velocity waves) for small systems: x (n x nrhs) lives in LDS; blocks of 64
+// rows are taken in solve order, each first reduced by the already solved part (dot products
+// over contiguous factor rows, 8 rows per wave) and then
 // solved with its diagonal block by substitution (wave 0, LAPACK dtrsv order).  Used when
 // the LDS holds x (n * nrhs * sizeof(T) <= kTrsvWgBytes).
 constexpr int kTrsvWgBytes = 64 * 1024;
+constexpr int kTrsvWgThreads = 512;
 template <typename T, bool LOWER>
-__global__ void __launch_bounds__(1024) trsv_wg_kernel(const T *__restrict__ A, int lda, int n, T *__restrict__ B,
+__global__ void __launch_bounds__(kTrsvWgThreads) __attribute__((amdgpu_waves_per_eu(5))) trsv_wg_kernel(const T *__restrict__ A, int lda, int n, T *__restrict__ B,
                                                       int nrhs)
 {
+    main_stream_priority<3>();
     extern __shared__ __align__(16) unsigned char smem_raw[];
     T *x = (T *)smem_raw; // n x nrhs
     __shared__ T Ds[NB][NB + 1];
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-    for (int i = tid; i < n * nrhs; i += 1024)
+    for (int i = tid; i < n * nrhs; i += kTrsvWgThreads)
         x[i] = B[i];
     __syncthreads();
     const int nblk = (n + NB - 1) / NB;
@@ -656,9 +813,11 @@ __global__ void __launch_bounds__(1024) trsv_wg_kernel(const T *__restrict__ A, 
         const int r0 = ib * NB, nr = min(NB, n - r0);
         // solved columns: [0, r0) for L, [r0 + nr, n) for U
         const int c0 = LOWER ? 0 : r0 + nr, c1 = LOWER ? r0 : n;
-        // 16 waves x 4 rows = the 64 rows of the block, the four rows' loads in flight together
+        // 8 waves x 2 passes x 4 rows = the 64 rows of the block, each pass's four rows' loads
+        // in flight together
+        for (int pass = 0; pass < 2; ++pass)
         {
-            const int ra = 4 * wv;
+            const int ra = 8 * wv + 4 * pass;
             const T *arow[4];
 #pragma unroll
             for (int j = 0; j < 4; ++j)
@@ -666,7 +825,7 @@ __global__ void __launch_bounds__(1024) trsv_wg_kernel(const T *__restrict__ A, 
             for (int q = 0; q < nrhs; ++q)
             {
                 T s4[4] = {T(0), T(0), T(0), T(0)};
-#pragma unroll 8
+#pragma unroll 4
                 for (int c = c0 + lane; c < c1; c += 64)
                 {
                     const T xv = x[c * nrhs + q];
@@ -687,7 +846,7 @@ __global__ void __launch_bounds__(1024) trsv_wg_kernel(const T *__restrict__ A, 
         }
         __syncthreads();
         // the diagonal block, staged in LDS by all threads (identity padding past n)
-        for (int idx = tid; idx < NB * NB; idx += 1024)
+        for (int idx = tid; idx < NB * NB; idx += kTrsvWgThreads)
         {
             const int r = idx / NB, c = idx % NB;
             Ds[r][c] = (r < nr && c < nr) ? A[(size_t)(r0 + r) * lda + r0 + c] : (r == c ? T(1) : T(0));
@@ -714,7 +873,7 @@ __global__ void __launch_bounds__(1024) trsv_wg_kernel(const T *__restrict__ A, 
         }
         __syncthreads();
     }
-    for (int i = tid; i < n * nrhs; i += 1024)
+    for (int i = tid; i < n * nrhs; i += kTrsvWgThreads)
         B[i] = x[i];
 }
 
@@ -724,6 +883,7 @@ __global__ void __launch_bounds__(256) residual_kernel(const TA *__restrict__ A,
                                                        const double *__restrict__ X, const double *__restrict__ B,
                                                        double *__restrict__ R, int nrhs)
 {
+    main_stream_priority();
     const int row = blockIdx.x * 4 + (threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
     if (row >= n)
@@ -758,6 +918,7 @@ __global__ void __launch_bounds__(256) residual_kernel(const TA *__restrict__ A,
 
 __global__ void f64_to_f32_kernel(const double *__restrict__ a, float *__restrict__ b, size_t count)
 {
+    main_stream_priority();
     for (size_t i = blockIdx.x * (size_t)256 + threadIdx.x; i < count; i += (size_t)gridDim.x * 256)
         b[i] = (float)a[i];
 }
@@ -767,6 +928,7 @@ __global__ void f64_to_f32_kernel(const double *__restrict__ a, float *__restric
 __global__ void __launch_bounds__(256) colabsmax_kernel(const double *__restrict__ M, int n, int nrhs,
                                                         double *__restrict__ out)
 {
+    main_stream_priority();
     const int q = blockIdx.x;
     double m = 0.0;
     for (int i = threadIdx.x; i < n; i += 256)
@@ -790,6 +952,7 @@ __global__ void __launch_bounds__(256) colabsmax_kernel(const double *__restrict
 // X += (double) D32
 __global__ void add_f32_kernel(double *__restrict__ X, const float *__restrict__ D, size_t count)
 {
+    main_stream_priority();
     for (size_t i = blockIdx.x * (size_t)256 + threadIdx.x; i < count; i += (size_t)gridDim.x * 256)
         X[i] = X[i] + (double)D[i];
 }
@@ -847,7 +1010,8 @@ template <typename T> int factor_impl(int n, T *A, int *piv, int *info_out, bool
         {
             T *P = A + (size_t)k * n + k;
             ProfScope prof(sizeof(T) == 8 ? "lu_panel_f64" : "lu_panel_f32");
-            panel_nopiv_kernel<T><<<1 + (m - jb + NB - 1) / NB, 256, 0, stream()>>>(P, n, m, jb, k, piv, status);
+            panel_nopiv_kernel<T><<<std::max(1, (m - jb + kNopivRows - 1) / kNopivRows), kNopivThreads, 0,
+                                     stream()>>>(P, n, m, jb, k, piv, status);
             PVL_HIP(hipGetLastError());
         }
         else if (nbf == kPanelWgCols && m <= panel_wg_rows(sizeof(T)))
@@ -891,9 +1055,13 @@ template <typename T> int factor_impl(int n, T *A, int *piv, int *info_out, bool
         const int n2 = n - k - jb;
         if (n2 > 0)
         {
+            if (pivoting) // the unpivoted panel kernel computes U12 itself
             {
                 ProfScope prof(sizeof(T) == 8 ? "lu_trsm_f64" : "lu_trsm_f32");
-                trsm_kernel<T><<<(n2 + NB - 1) / NB, 256, 0, stream()>>>(A, n, n, k, jb);
+                if (jb == NB)
+                    trsm_kernel<T, NB><<<(n2 + 63) / 64, 256, 0, stream()>>>(A, n, n, k);
+                else
+                    trsm_kernel<T, kPanelWgCols><<<(n2 + 63) / 64, 256, 0, stream()>>>(A, n, n, k);
                 PVL_HIP(hipGetLastError());
             }
             const dim3 grid((n2 + 63) / 64, (n2 + 63) / 64);
@@ -975,8 +1143,8 @@ template <typename T> int solve_impl(pvl_lu *h, T *B, int nrhs)
         {
             ProfScope prof(sizeof(T) == 8 ? "lu_trsv_f64" : "lu_trsv_f32");
             const T *Ap = (const T *)h->lu;
-            trsv_wg_kernel<T, true><<<1, 1024, xbytes, stream()>>>(Ap, n, n, work, nq);
-            trsv_wg_kernel<T, false><<<1, 1024, xbytes, stream()>>>(Ap, n, n, work, nq);
+            trsv_wg_kernel<T, true><<<1, kTrsvWgThreads, xbytes, stream()>>>(Ap, n, n, work, nq);
+            trsv_wg_kernel<T, false><<<1, kTrsvWgThreads, xbytes, stream()>>>(Ap, n, n, work, nq);
             PVL_HIP(hipGetLastError());
         }
         else
@@ -1260,6 +1428,7 @@ int pvl_convert_f64_f32(const double *src, float *dst, size_t count)
 // body's block; removes the uniform-circulation null direction of a closed surface).
 __global__ void deflate_kernel(double *A, int n, int r0, int r1, double c)
 {
+    main_stream_priority();
     const int len = r1 - r0;
     for (size_t idx = blockIdx.x * (size_t)256 + threadIdx.x; idx < (size_t)len * len;
          idx += (size_t)gridDim.x * 256)

@@ -30,6 +30,7 @@ template <typename T, int NBP, int RPT>
 __global__ void __launch_bounds__(512) panel_wg_kernel(T *__restrict__ P, int lda, int m, int jb, int row_base,
                                                        int *__restrict__ piv, int *__restrict__ status)
 {
+    main_stream_priority<2>();
     static_assert((NBP & (NBP - 1)) == 0, "NBP must be a power of two");
     constexpr int NT = 512;
     constexpr unsigned NONE = 0xFFFFFFFFu;

@@ -18,16 +18,24 @@ constexpr uint32_t kReversed = 0x80000000u;  // src/core/common.h:55
 // Error state (thread local) and the library stream.
 void set_error(const char *fmt, ...);
 hipStream_t stream();
+hipStream_t side_stream(); // nullptr (error set) if it cannot be created
+int side_groups();         // workgroup cap for grid-stride kernels on the side stream
+
+// Kernels on the library stream's critical path raise their waves' issue priority above
+// the side stream's (priority 0), which shares their SIMDs.
+template <int LEVEL = 1> __device__ __forceinline__ void main_stream_priority() { __builtin_amdgcn_s_setprio(LEVEL); }
 int ensure_init();
 
-// Kernel timing hooks: when profiling is on, brackets the launches between begin/end with
-// HIP events on the library stream and accumulates the elapsed time under `name`.
+// Kernel timing hook: when profiling is on, brackets the launches made on stream s (default:
+// the library stream) during the scope's lifetime with HIP events and accumulates the
+// elapsed time under `name`.
 struct ProfScope
 {
     const char *name;
     double work;
+    hipStream_t strm;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
-    explicit ProfScope(const char *n, double work_units = 0.0);
+    explicit ProfScope(const char *n, double work_units = 0.0, hipStream_t s = nullptr);
     ~ProfScope();
 };
 
@@ -54,6 +62,20 @@ namespace pvl
 int point_velocity_impl(const pvl_mesh *m, double tol, const double *pos, const double *circ, unsigned n_pts,
                         const double *pts, double *out, int accumulate, int precision, unsigned n_active,
                         const char *prof_name = "point_velocity");
+
+// The same sum in two parts with caller-owned buffers and stream.  point_velocity_chunks
+// gives the line-chunk count (n_pts, n_active > 0); point_velocity_partials writes the
+// per-chunk partial sums (geom: n_active LineGeom, gamma: n_active doubles, partial:
+// 3 n_pts n_chunks doubles) on stream s with at most max_groups workgroups (0: one per work
+// item); point_velocity_reduce adds them, chunk by chunk in order, to out (or to 0 when
+// accumulate is 0) on stream s.
+unsigned point_velocity_chunks(unsigned n_pts, unsigned n_active);
+size_t line_geom_bytes(unsigned n_lines);
+int point_velocity_partials(const pvl_mesh *m, double tol, const double *pos, const double *circ, unsigned n_pts,
+                            const double *pts, int precision, unsigned n_active, const char *prof_name,
+                            hipStream_t s, void *geom, double *gamma, double *partial, unsigned max_groups);
+int point_velocity_reduce(unsigned n_pts, unsigned n_chunks, const double *partial, double *out, int accumulate,
+                          hipStream_t s);
 
 // Single-workgroup LU panel (lu_panel_f64.hip / lu_panel_f32.hip): factors the m x jb panel
 // at P (row stride lda; matrix rows row_base..) with partial pivoting, piv[row_base + j]

@@ -3,6 +3,7 @@
 #include "pvl_internal.hpp"
 
 #include <cstdarg>
+#include <cstdlib>
 #include <map>
 #include <mutex>
 #include <string>
@@ -77,6 +78,39 @@ int ensure_init()
 }
 
 hipStream_t stream() { return g_stream; }
+
+// Workgroups per CU given to work on the side stream (environment PVL_SIDE_GROUPS_PER_CU,
+// default 4): the rest of each CU's slots stay free for the library stream's kernels.
+int side_groups()
+{
+    static int groups = -1;
+    if (groups < 0)
+    {
+        int per_cu = 4;
+        if (const char *e = getenv("PVL_SIDE_GROUPS_PER_CU"))
+            per_cu = atoi(e) > 0 ? atoi(e) : per_cu;
+        int cus = 0;
+        if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, g_device < 0 ? 0 : g_device) !=
+                hipSuccess ||
+            cus <= 0)
+            cus = 256;
+        groups = per_cu * cus;
+    }
+    return groups;
+}
+
+// Second library stream (default priority) for work that runs beside the library stream;
+// ordered against it with events only.
+static hipStream_t g_side = nullptr;
+hipStream_t side_stream()
+{
+    if (g_side == nullptr && hipStreamCreateWithFlags(&g_side, hipStreamNonBlocking) != hipSuccess)
+    {
+        set_error("hipStreamCreateWithFlags(side stream) failed");
+        g_side = nullptr;
+    }
+    return g_side;
+}
 
 // ---------------------------------------------------------------------------------------
 // Scratch buffers: grown on demand, never shrunk; freed at process end by the runtime.
@@ -158,13 +192,14 @@ static void resolve_pending()
     g_pending.clear();
 }
 
-ProfScope::ProfScope(const char *n, double work_units) : name(n), work(work_units)
+ProfScope::ProfScope(const char *n, double work_units, hipStream_t s)
+    : name(n), work(work_units), strm(s ? s : g_stream)
 {
     if (!g_prof || (!g_prof_only.empty() && g_prof_only != n))
         return;
     ev0 = take_event();
     ev1 = take_event();
-    if (!ev0 || !ev1 || hipEventRecord(ev0, g_stream) != hipSuccess)
+    if (!ev0 || !ev1 || hipEventRecord(ev0, strm) != hipSuccess)
     {
         if (ev0)
             g_event_pool.push_back(ev0);
@@ -178,7 +213,7 @@ ProfScope::~ProfScope()
 {
     if (!ev0)
         return;
-    if (hipEventRecord(ev1, g_stream) != hipSuccess)
+    if (hipEventRecord(ev1, strm) != hipSuccess)
     {
         g_event_pool.push_back(ev0);
         g_event_pool.push_back(ev1);

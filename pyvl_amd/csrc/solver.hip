@@ -11,6 +11,7 @@ namespace
 __global__ void rhs_kernel(unsigned n, const double *__restrict__ normals, const double *__restrict__ cpvel,
                            double vx, double vy, double vz, double *__restrict__ out)
 {
+    main_stream_priority();
     const unsigned i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n)
         return;
@@ -24,6 +25,7 @@ __global__ void rhs_kernel(unsigned n, const double *__restrict__ normals, const
 // x[r0:r1] -= mean(x[r0:r1]) in one workgroup (python/pyvl/solver.py:229-233)
 __global__ void __launch_bounds__(1024) remove_mean_kernel(double *__restrict__ x, int r0, int r1)
 {
+    main_stream_priority();
     __shared__ double part[1024];
     double s = 0.0;
     for (int i = r0 + threadIdx.x; i < r1; i += 1024)

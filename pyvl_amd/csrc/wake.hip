@@ -48,6 +48,16 @@ struct pvl_wake
     double *d_circ_x = nullptr;        // [R*n_shed] explicit-part circulation (row 0 zeroed)
     pvl_mesh *full = nullptr;          // all rings, row-major
     pvl_mesh *row0 = nullptr;          // ring row 0 only
+    // Self-induction partial sums started early on the side stream (pvl_wake_begin_update),
+    // consumed by the next pvl_wake_update.
+    void *d_sgeom = nullptr;           // [full->n_lines] LineGeom
+    double *d_sgamma = nullptr;        // [full->n_lines]
+    double *d_spartial = nullptr;      // [3 * n_pts * chunks]
+    size_t spartial_bytes = 0;
+    hipEvent_t ev_ready = nullptr, ev_done = nullptr;
+    bool pending = false;
+    int pending_prec = 0;
+    unsigned pending_na = 0, pending_npts = 0, pending_chunks = 0;
 };
 
 namespace
@@ -56,6 +66,7 @@ namespace
 __global__ void pin_row0_kernel(double *__restrict__ upos, unsigned n_te, const double *__restrict__ bound_pos,
                                 const uint32_t *__restrict__ te_nodes)
 {
+    main_stream_priority();
     const unsigned i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n_te)
         return;
@@ -70,6 +81,7 @@ __global__ void advect_kernel(double *__restrict__ upos, const double *__restric
                               const double *__restrict__ vel, unsigned n, unsigned n_te, double vx, double vy,
                               double vz, double dt)
 {
+    main_stream_priority();
     const unsigned i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n)
         return;
@@ -83,6 +95,7 @@ __global__ void advect_kernel(double *__restrict__ upos, const double *__restric
 // rows cr+1..R copy row cr (python/pyvl/wake_models.py:134-137)
 __global__ void fill_rows_kernel(double *__restrict__ upos, unsigned n_te, unsigned cr, unsigned R)
 {
+    main_stream_priority();
     const unsigned i = blockIdx.x * blockDim.x + threadIdx.x;
     const unsigned n = (R - cr) * n_te;
     if (i >= n)
@@ -96,6 +109,7 @@ __global__ void fill_rows_kernel(double *__restrict__ upos, unsigned n_te, unsig
 __global__ void shed_kernel(const double *__restrict__ old, double *__restrict__ neu, unsigned n_shed, unsigned R,
                             const uint32_t *__restrict__ adj, const double *__restrict__ gamma)
 {
+    main_stream_priority();
     const unsigned i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n_shed * R)
         return;
@@ -117,6 +131,7 @@ __global__ void shed_kernel(const double *__restrict__ old, double *__restrict__
 __global__ void implicit_kernel(double *__restrict__ A, unsigned n_cpts, unsigned n_surf,
                                 const double *__restrict__ w, unsigned n_shed, const uint32_t *__restrict__ adj)
 {
+    main_stream_priority();
     const unsigned i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n_cpts)
         return;
@@ -137,6 +152,7 @@ __global__ void implicit_kernel(double *__restrict__ A, unsigned n_cpts, unsigne
 __global__ void explicit_kernel(double *__restrict__ b, const double *__restrict__ normals,
                                 const double *__restrict__ v, unsigned n_cpts)
 {
+    main_stream_priority();
     const unsigned i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n_cpts)
         return;
@@ -308,6 +324,16 @@ int pvl_wake_destroy(pvl_wake *w)
     if (!w)
         return PVL_OK;
     (void)hipStreamSynchronize(stream());
+    if (w->ev_done)
+    {
+        (void)hipEventSynchronize(w->ev_done);
+        (void)hipEventDestroy(w->ev_done);
+    }
+    if (w->ev_ready)
+        (void)hipEventDestroy(w->ev_ready);
+    (void)hipFree(w->d_sgeom);
+    (void)hipFree(w->d_sgamma);
+    (void)hipFree(w->d_spartial);
     pvl_mesh_destroy(w->full);
     pvl_mesh_destroy(w->row0);
     (void)hipFree(w->d_te_nodes);
@@ -346,6 +372,71 @@ static unsigned active_lines(const pvl_wake *w)
     return w->nl_prefix[rows];
 }
 
+// Orders the library stream after a started self-induction (its inputs may be overwritten
+// from here on) and drops it.
+static int drop_pending(pvl_wake *w)
+{
+    if (!w->pending)
+        return PVL_OK;
+    w->pending = false;
+    PVL_HIP(hipStreamWaitEvent(stream(), w->ev_done, 0));
+    return PVL_OK;
+}
+
+int pvl_wake_begin_update(pvl_wake *w, int mode, int precision)
+{
+    if (int st = ensure_init())
+        return st;
+    if (mode != 0 && mode != 1)
+    {
+        set_error("wake mode must be 0 (reference) or 1 (free wake)");
+        return PVL_ERR_ARG;
+    }
+    if (precision != PVL_FP64 && precision != PVL_FP32 && precision != PVL_FP64_REF)
+    {
+        set_error("unknown precision %d", precision);
+        return PVL_ERR_ARG;
+    }
+    const unsigned na = active_lines(w);
+    if (mode != 1 || na == 0 || w->pending)
+        return PVL_OK;
+    const unsigned cr = w->step_count < w->R ? w->step_count : w->R;
+    const unsigned n_old = cr * w->n_te;
+    hipStream_t s = side_stream();
+    if (!s)
+        return PVL_ERR_HIP;
+    if (!w->ev_ready)
+        PVL_HIP(hipEventCreateWithFlags(&w->ev_ready, hipEventDisableTiming));
+    if (!w->ev_done)
+        PVL_HIP(hipEventCreateWithFlags(&w->ev_done, hipEventDisableTiming));
+    if (!w->d_sgeom)
+        PVL_HIP(hipMalloc(&w->d_sgeom, line_geom_bytes(w->full->n_lines)));
+    if (!w->d_sgamma)
+        PVL_HIP(hipMalloc((void **)&w->d_sgamma, sizeof(double) * (w->full->n_lines ? w->full->n_lines : 1)));
+    const unsigned nc = point_velocity_chunks(n_old, na);
+    const size_t pbytes = sizeof(double) * 3 * (size_t)n_old * nc;
+    if (pbytes > w->spartial_bytes)
+    {
+        // the previous buffer is no longer read: every started sum was consumed or dropped
+        (void)hipFree(w->d_spartial);
+        w->d_spartial = nullptr, w->spartial_bytes = 0;
+        PVL_HIP(hipMalloc((void **)&w->d_spartial, pbytes));
+        w->spartial_bytes = pbytes;
+    }
+    // the wake state as the library stream leaves it now; the points are the first n_old
+    // wake nodes (pvl_wake_update evaluates at a copy of the same values)
+    PVL_HIP(hipEventRecord(w->ev_ready, stream()));
+    PVL_HIP(hipStreamWaitEvent(s, w->ev_ready, 0));
+    if (int st = point_velocity_partials(w->full, w->tol, w->d_upos, w->d_circ[w->cur], n_old, w->d_upos, precision,
+                                         na, "wake_self_velocity", s, w->d_sgeom, w->d_sgamma, w->d_spartial,
+                                         (unsigned)side_groups()))
+        return st;
+    PVL_HIP(hipEventRecord(w->ev_done, s));
+    w->pending = true;
+    w->pending_prec = precision, w->pending_na = na, w->pending_npts = n_old, w->pending_chunks = nc;
+    return PVL_OK;
+}
+
 int pvl_wake_update(pvl_wake *w, double time, const pvl_mesh *bound, const double *bound_pos_dev,
                     const double *circulation_dev, const double *vinf, int mode, int precision)
 {
@@ -376,13 +467,25 @@ int pvl_wake_update(pvl_wake *w, double time, const pvl_mesh *bound, const doubl
     if (mode == 1)
     {
         const unsigned na = active_lines(w);
-        if (na)
+        if (na && w->pending && w->pending_prec == precision && w->pending_na == na && w->pending_npts == n_old)
         {
+            // started by pvl_wake_begin_update: the same partial sums, added in the same order
+            w->pending = false;
+            PVL_HIP(hipStreamWaitEvent(stream(), w->ev_done, 0));
+            if (int st = point_velocity_reduce(n_old, w->pending_chunks, w->d_spartial, w->d_vel, 1, stream()))
+                return st;
+        }
+        else if (na)
+        {
+            if (int st = drop_pending(w))
+                return st;
             if (int st = point_velocity_impl(w->full, w->tol, w->d_upos, w->d_circ[w->cur], n_old, w->d_old, w->d_vel,
                                              1, precision, na, "wake_self_velocity"))
                 return st;
         }
     }
+    if (int st = drop_pending(w))
+        return st;
     advect_kernel<<<(n_old + 255) / 256, 256, 0, stream()>>>(w->d_upos, w->d_old, w->d_vel, n_old, n_te, vinf[0],
                                                              vinf[1], vinf[2], dt);
     PVL_HIP(hipGetLastError());
@@ -516,6 +619,8 @@ int pvl_wake_set_state(pvl_wake *w, const double *positions_host, const double *
     for (unsigned s = 0; s < n_shed; ++s)
         for (unsigned r = 0; r < R; ++r)
             ci[(size_t)r * n_shed + s] = circulation_host[(size_t)s * R + r];
+    if (int st = drop_pending(w))
+        return st;
     PVL_HIP(hipMemcpyAsync(w->d_upos, up.data(), sizeof(double) * up.size(), hipMemcpyHostToDevice, stream()));
     PVL_HIP(hipMemcpyAsync(w->d_circ[w->cur], ci.data(), sizeof(double) * ci.size(), hipMemcpyHostToDevice,
                            stream()));

@@ -178,6 +178,9 @@ class UnsteadySolver:
         self.gmres_tol = 1e-12
         self.gmres_stats: list[tuple] = []
         self.fallbacks = 0
+        # start the free wake's self-induction at the beginning of each step, beside the
+        # bound solve (bit-identical results; False: strictly in the reference's order)
+        self.overlap_wake = True
         self.d_b0 = DeviceArray((n,)) if solve == "gmres" else None
         self.d_A32 = DeviceArray((n, n), np.float32) if solve == "gmres" else None
 
@@ -237,6 +240,9 @@ class UnsteadySolver:
 
     def step(self) -> DeviceArray:
         t = self.iteration * self.dt
+        if self.overlap_wake and self.wake is not None:
+            # the wake's self-induction needs only the wake state: it runs beside the solve
+            self.wake.begin_update()
         self.assemble(t)
         self.solve_system()
         if self.wake is not None:

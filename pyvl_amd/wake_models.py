@@ -116,6 +116,12 @@ class WakeModelLineExplicitUnsteady:
         _lib.call("pvl_wake_update", self._h, C.c_double(float(time)), mesh._device(), pos.cptr, circ.cptr,
                   v.ctypes.data_as(C.POINTER(C.c_double)), C.c_int(_MODES[self.mode]), C.c_int(self.precision))
 
+    def begin_update(self) -> None:
+        """Start the next update's wake self-induction on a second stream (free wake only; no
+        reference counterpart: the results of :meth:`update` are unchanged, it may only finish
+        sooner because the work overlaps what is issued in between)."""
+        _lib.call("pvl_wake_begin_update", self._h, C.c_int(_MODES[self.mode]), C.c_int(self.precision))
+
     def get_velocity(self, positions):
         """Velocity induced by the wake at ``positions`` (python/pyvl/wake_models.py:159-175)."""
         if isinstance(positions, DeviceArray):
