@@ -242,17 +242,23 @@ __global__ void __launch_bounds__(256) line_circulation_kernel(unsigned n_lines,
     gamma[i] = g;
 }
 
-// One lane per point; the workgroup's four waves split a chunk of lines, partial sums
-// are combined in LDS in wave order and written per chunk; a second pass adds the chunks
-// in chunk order.  No atomics: the result does not depend on scheduling.  Work items are
-// (64-point block, line chunk) pairs, item = chunk * pblocks + block; workgroup b takes items
-// b, b + gridDim.x, .. (a grid smaller than the item count leaves CU slots to other work on
-// the device; each item's arithmetic is the same either way).
-// SITE only gives the free-wake self-induction launch its own symbol for profilers.
-// At most 80 VGPRs (6 waves per SIMD): with 4 workgroups per CU on the side stream, each
-// SIMD keeps 192 VGPRs for the library stream's kernels.
+// PPL points per lane (64 PPL per wave, point j * 64 + lane of the block); the workgroup's
+// four waves split a chunk of lines, partial sums are combined in LDS in wave order and
+// written per chunk; a second pass adds the chunks in chunk order.  No atomics: the result
+// does not depend on scheduling.  Work items are (64 PPL-point block, line chunk) pairs,
+// item = chunk * pblocks + block; workgroup b takes items b, b + gridDim.x, .. (a grid
+// smaller than the item count leaves CU slots to other work on the device; each item's
+// arithmetic is the same either way).  With PPL = 2 each lane runs two independent
+// evaluation chains per line (instruction-level parallelism) and every line's geometry,
+// loaded once into scalar registers, serves twice the points.
+// SITE only gives the free-wake self-induction launch its own symbol for profilers; it
+// uses PPL = 2, the other sites PPL = 1.  The self-induction kernel needs 60 VGPRs: five
+// workgroups per CU on the side stream (one wave each per SIMD) leave each SIMD 192 VGPRs
+// for the library stream's kernels.
+template <int SITE> constexpr int site_ppl() { return SITE == 1 ? 2 : 1; }
+
 template <int PREC, int SITE>
-__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6)))
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(SITE == 1 ? 4 : 6)))
     point_velocity_kernel(unsigned n_pts, const double *__restrict__ pts, unsigned n_lines,
                           const LineGeom *__restrict__ geom, const double *__restrict__ gamma,
                           unsigned lines_per_chunk, double *__restrict__ partial, double tol, unsigned pblocks,
@@ -260,40 +266,65 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6))
 {
     if constexpr (SITE == 0)
         main_stream_priority();
-    __shared__ double red[4][3][kWave];
+    constexpr int PPL = site_ppl<SITE>();
+    __shared__ double red[4][3][kWave * PPL];
     const unsigned w = wave_id(), lane = lane_id();
     const double tol2 = tol * tol;
     const unsigned per_wave = (lines_per_chunk + 3) / 4;
     for (unsigned item = blockIdx.x; item < n_items; item += gridDim.x)
     {
         const unsigned chunk = item / pblocks;
-        const unsigned p = (item - chunk * pblocks) * kWave + lane;
-        const unsigned pc = p < n_pts ? p : n_pts - 1;
-        const double px = pts[3 * pc + 0], py = pts[3 * pc + 1], pz = pts[3 * pc + 2];
+        const unsigned p0 = (item - chunk * pblocks) * (kWave * PPL) + lane;
+        double px[PPL], py[PPL], pz[PPL], vx[PPL], vy[PPL], vz[PPL];
+#pragma unroll
+        for (int j = 0; j < PPL; ++j)
+        {
+            const unsigned p = p0 + j * kWave;
+            const unsigned pc = p < n_pts ? p : n_pts - 1;
+            px[j] = pts[3 * pc + 0], py[j] = pts[3 * pc + 1], pz[j] = pts[3 * pc + 2];
+            vx[j] = 0.0, vy[j] = 0.0, vz[j] = 0.0;
+        }
         const unsigned l0 = chunk * lines_per_chunk;
         const unsigned l1 = (l0 + lines_per_chunk) < n_lines ? (l0 + lines_per_chunk) : n_lines;
         const unsigned a = l0 + w * per_wave;
         const unsigned b = (a + per_wave) < l1 ? (a + per_wave) : l1;
-        double vx = 0.0, vy = 0.0, vz = 0.0;
         for (unsigned l = a; l < b; ++l)
-            line_k_acc<PREC>(geom[l], px, py, pz, tol, tol2, gamma[l], vx, vy, vz);
-        red[w][0][lane] = vx;
-        red[w][1][lane] = vy;
-        red[w][2][lane] = vz;
-        __syncthreads();
-        if (w == 0 && p < n_pts)
         {
-            double sx = red[0][0][lane], sy = red[0][1][lane], sz = red[0][2][lane];
-            for (int q = 1; q < 4; ++q)
+            const LineGeom g = geom[l];
+            const double gl = gamma[l];
+#pragma unroll
+            for (int j = 0; j < PPL; ++j)
+                line_k_acc<PREC>(g, px[j], py[j], pz[j], tol, tol2, gl, vx[j], vy[j], vz[j]);
+        }
+#pragma unroll
+        for (int j = 0; j < PPL; ++j)
+        {
+            red[w][0][j * kWave + lane] = vx[j];
+            red[w][1][j * kWave + lane] = vy[j];
+            red[w][2][j * kWave + lane] = vz[j];
+        }
+        __syncthreads();
+        if (w == 0)
+        {
+#pragma unroll
+            for (int j = 0; j < PPL; ++j)
             {
-                sx = sx + red[q][0][lane];
-                sy = sy + red[q][1][lane];
-                sz = sz + red[q][2][lane];
+                const unsigned p = p0 + j * kWave;
+                if (p >= n_pts)
+                    continue;
+                const unsigned i = j * kWave + lane;
+                double sx = red[0][0][i], sy = red[0][1][i], sz = red[0][2][i];
+                for (int q = 1; q < 4; ++q)
+                {
+                    sx = sx + red[q][0][i];
+                    sy = sy + red[q][1][i];
+                    sz = sz + red[q][2][i];
+                }
+                double *dst = partial + ((size_t)chunk * n_pts + p) * 3;
+                dst[0] = sx;
+                dst[1] = sy;
+                dst[2] = sz;
             }
-            double *dst = partial + ((size_t)chunk * n_pts + p) * 3;
-            dst[0] = sx;
-            dst[1] = sy;
-            dst[2] = sz;
         }
         __syncthreads(); // red is rewritten by the next item
     }
@@ -464,9 +495,15 @@ namespace pvl
 // entry point; the wake passes the prefix of lines its filled rows use -- the other lines
 // carry zero circulation).
 // Chunking: aim for >= ~4 workgroups per CU while keeping >= 64 lines per wave.
-static void plan_chunks(unsigned n_pts, unsigned n_active, unsigned *n_chunks_out, unsigned *lpc_out)
+static bool self_site(const char *prof_name) { return std::strcmp(prof_name, "wake_self_velocity") == 0; }
+static unsigned site_points_per_group(const char *prof_name)
 {
-    const unsigned pblocks = (n_pts + kWave - 1) / kWave;
+    return kWave * (self_site(prof_name) ? site_ppl<1>() : site_ppl<0>());
+}
+
+static void plan_chunks(unsigned n_pts, unsigned n_active, unsigned ppg, unsigned *n_chunks_out, unsigned *lpc_out)
+{
+    const unsigned pblocks = (n_pts + ppg - 1) / ppg;
     unsigned n_chunks = (2048 + pblocks - 1) / pblocks;
     const unsigned max_chunks = (n_active + 255) / 256;
     if (n_chunks > max_chunks)
@@ -480,10 +517,10 @@ static void plan_chunks(unsigned n_pts, unsigned n_active, unsigned *n_chunks_ou
     *lpc_out = lpc;
 }
 
-unsigned point_velocity_chunks(unsigned n_pts, unsigned n_active)
+unsigned point_velocity_chunks(unsigned n_pts, unsigned n_active, const char *prof_name)
 {
     unsigned nc, lpc;
-    plan_chunks(n_pts, n_active, &nc, &lpc);
+    plan_chunks(n_pts, n_active, site_points_per_group(prof_name), &nc, &lpc);
     return nc;
 }
 
@@ -500,14 +537,14 @@ int point_velocity_partials(const pvl_mesh *m, double tol, const double *pos, co
                                                                      circ, gamma);
     PVL_HIP(hipGetLastError());
     unsigned n_chunks, lpc;
-    plan_chunks(n_pts, n_active, &n_chunks, &lpc);
-    const unsigned pblocks = (n_pts + kWave - 1) / kWave;
+    const unsigned ppg = site_points_per_group(prof_name);
+    plan_chunks(n_pts, n_active, ppg, &n_chunks, &lpc);
+    const unsigned pblocks = (n_pts + ppg - 1) / ppg;
     const unsigned n_items = pblocks * n_chunks;
     const unsigned grid = max_groups && max_groups < n_items ? max_groups : n_items;
     ProfScope prof(prof_name, (double)n_pts * (double)n_active, s);
-    const bool self_site = std::strcmp(prof_name, "wake_self_velocity") == 0;
     dispatch_precision(precision, [&](auto P) {
-        if (self_site)
+        if (self_site(prof_name))
             point_velocity_kernel<decltype(P)::value, 1>
                 <<<grid, kBlock, 0, s>>>(n_pts, pts, n_active, g, gamma, lpc, partial, tol, pblocks, n_items);
         else
@@ -540,7 +577,7 @@ int point_velocity_impl(const pvl_mesh *m, double tol, const double *pos, const 
             PVL_HIP(hipMemsetAsync(out, 0, sizeof(double) * 3 * n_pts, stream()));
         return PVL_OK;
     }
-    const unsigned n_chunks = point_velocity_chunks(n_pts, n_active);
+    const unsigned n_chunks = point_velocity_chunks(n_pts, n_active, prof_name);
     void *g = scratch(0, line_geom_bytes(n_active));
     double *gamma = (double *)scratch(1, sizeof(double) * n_active);
     double *partial = (double *)scratch(2, sizeof(double) * 3 * (size_t)n_pts * n_chunks);

@@ -64,12 +64,13 @@ int point_velocity_impl(const pvl_mesh *m, double tol, const double *pos, const 
                         const char *prof_name = "point_velocity");
 
 // The same sum in two parts with caller-owned buffers and stream.  point_velocity_chunks
-// gives the line-chunk count (n_pts, n_active > 0); point_velocity_partials writes the
+// gives the line-chunk count (n_pts, n_active > 0; the chunking depends on the launch
+// site prof_name, as the kernel's points per lane do); point_velocity_partials writes the
 // per-chunk partial sums (geom: n_active LineGeom, gamma: n_active doubles, partial:
 // 3 n_pts n_chunks doubles) on stream s with at most max_groups workgroups (0: one per work
 // item); point_velocity_reduce adds them, chunk by chunk in order, to out (or to 0 when
 // accumulate is 0) on stream s.
-unsigned point_velocity_chunks(unsigned n_pts, unsigned n_active);
+unsigned point_velocity_chunks(unsigned n_pts, unsigned n_active, const char *prof_name);
 size_t line_geom_bytes(unsigned n_lines);
 int point_velocity_partials(const pvl_mesh *m, double tol, const double *pos, const double *circ, unsigned n_pts,
                             const double *pts, int precision, unsigned n_active, const char *prof_name,

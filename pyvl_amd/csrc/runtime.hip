@@ -80,13 +80,13 @@ int ensure_init()
 hipStream_t stream() { return g_stream; }
 
 // Workgroups per CU given to work on the side stream (environment PVL_SIDE_GROUPS_PER_CU,
-// default 4): the rest of each CU's slots stay free for the library stream's kernels.
+// default 5): the rest of each CU's slots stay free for the library stream's kernels.
 int side_groups()
 {
     static int groups = -1;
     if (groups < 0)
     {
-        int per_cu = 4;
+        int per_cu = 5;
         if (const char *e = getenv("PVL_SIDE_GROUPS_PER_CU"))
             per_cu = atoi(e) > 0 ? atoi(e) : per_cu;
         int cus = 0;

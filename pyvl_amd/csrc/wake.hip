@@ -413,7 +413,7 @@ int pvl_wake_begin_update(pvl_wake *w, int mode, int precision)
         PVL_HIP(hipMalloc(&w->d_sgeom, line_geom_bytes(w->full->n_lines)));
     if (!w->d_sgamma)
         PVL_HIP(hipMalloc((void **)&w->d_sgamma, sizeof(double) * (w->full->n_lines ? w->full->n_lines : 1)));
-    const unsigned nc = point_velocity_chunks(n_old, na);
+    const unsigned nc = point_velocity_chunks(n_old, na, "wake_self_velocity");
     const size_t pbytes = sizeof(double) * 3 * (size_t)n_old * nc;
     if (pbytes > w->spartial_bytes)
     {
