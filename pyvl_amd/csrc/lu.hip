@@ -332,9 +332,11 @@ template <int P> __device__ __forceinline__ float quad_bcast(float v)
 // chain and run at the highest priority; the other two waves fill in behind them.  Per element the
 // operations and their order are those of the right-looking column sweep (dgetf2 without
 // interchanges, dtrsm).  The copies of the diagonal block agree (same arithmetic
-// everywhere); workgroup 0 stores L11 \ U11 and the identity interchanges.  Only the last
-// panel can be narrower than 64 (jb < 64 implies m == jb: no rows below, no columns right):
-// it is padded with identity rows and columns.
+// everywhere).  Every workgroup reads A11 at its start, so L11 \ U11 is stored by the LAST
+// workgroup to finish (status[4] counts finished workgroups and is reset to 0 by that
+// workgroup; no workgroup waits for another); workgroup 0 stores the identity interchanges.
+// Only the last panel can be narrower than 64 (jb < 64 implies m == jb: no rows below, no
+// columns right): it is padded with identity rows and columns.
 // status[2]: smallest zero-pivot column (atomicMin); status[3] |= 1 when partial pivoting
 // would interchange: a sub-diagonal |a| above |pivot| at its column's elimination, or a zero
 // or NaN pivot.
@@ -491,17 +493,12 @@ __global__ void __launch_bounds__(kNopivThreads) __attribute__((amdgpu_waves_per
     }
     nopiv_publish<0>(L);
     nopiv_steps(L, std::make_integer_sequence<int, NB>{});
-#pragma unroll
-    for (int k = 0; k < KQ; ++k)
+    if (L.role != 0 && L.live)
     {
-        const int c = 4 * k + p;
-        if (L.role == 0)
+#pragma unroll
+        for (int k = 0; k < KQ; ++k)
         {
-            if (g == 0 && rd < jb && c < jb)
-                P[(size_t)rd * lda + c] = L.v[k];
-        }
-        else if (L.live)
-        {
+            const int c = 4 * k + p;
             if (L.role == 1)
                 P[(size_t)rc * lda + c] = L.v[k];
             else
@@ -517,6 +514,25 @@ __global__ void __launch_bounds__(kNopivThreads) __attribute__((amdgpu_waves_per
     }
     if (__ballot(L.due) && lane == 0)
         atomicOr(&status[3], 1);
+    // This workgroup's read of A11 is complete (its values fed every step).  The workgroup
+    // that finishes last stores L11 \ U11 over A11 and resets the counter.
+    __shared__ int last;
+    __syncthreads();
+    if (threadIdx.x == 0)
+        last = atomicAdd(&status[4], 1) == (int)gridDim.x - 1;
+    __syncthreads();
+    if (last && L.role == 0)
+    {
+#pragma unroll
+        for (int k = 0; k < KQ; ++k)
+        {
+            const int c = 4 * k + p;
+            if (rd < jb && c < jb)
+                P[(size_t)rd * lda + c] = L.v[k];
+        }
+        if (threadIdx.x == 0)
+            atomicExch(&status[4], 0);
+    }
 }
 
 // Row interchanges of one panel applied to every column outside it.
@@ -962,7 +978,8 @@ __global__ void add_f32_kernel(double *__restrict__ X, const float *__restrict__
 struct Workspace
 {
     void *panelx = nullptr;
-    int *err = nullptr; // [0]: first zero pivot, [1]: a bounded wait timed out, [2]: no-pivot zero pivot
+    int *err = nullptr; // [0]: first zero pivot, [1]: a bounded wait timed out, [2]: no-pivot zero pivot,
+                        // [3]: an interchange was due, [4]: workgroups done (unpivoted panel)
 };
 Workspace g_ws;
 
@@ -971,8 +988,8 @@ int ensure_ws()
     if (!g_ws.panelx)
     {
         PVL_HIP(hipMalloc(&g_ws.panelx, sizeof(PanelX<double>)));
-        PVL_HIP(hipMalloc(&g_ws.err, 4 * sizeof(int)));
-        PVL_HIP(hipMemset(g_ws.err, 0, 4 * sizeof(int)));
+        PVL_HIP(hipMalloc(&g_ws.err, 8 * sizeof(int)));
+        PVL_HIP(hipMemset(g_ws.err, 0, 8 * sizeof(int)));
     }
     return PVL_OK;
 }

@@ -221,7 +221,8 @@ def test_unpivoted_fp32_refinement_closed_body():
         assert _rel(x.numpy()[:, q], x_ref[:, q]) < 1e-10
 
 
-@pytest.mark.parametrize("n,dominant", [(300, True), (1000, True), (2100, True), (300, False), (1000, False), (2100, False)])
+@pytest.mark.parametrize("n,dominant", [(300, True), (1000, True), (2100, True), (6000, True), (300, False), (1000, False),
+                                        (2100, False)])
 def test_auto_pivoting_is_partial_pivoting(n, dominant):
     """"auto" gives partial pivoting's factorisation: identity pivots and the unpivoted
     factors when no interchange is due (diagonally dominant), LAPACK's pivots and factors
@@ -249,3 +250,23 @@ def test_auto_pivoting_interchange_below_the_diagonal_block():
     h = LA.LU(n, LA.FP64, pivoting="auto").factor(a)
     assert np.array_equal(h.pivots(), piv_ref)
     assert _rel(h.factors(), lu_ref) < 1e-12
+
+
+def test_unpivoted_fp32_refinement_large():
+    """FP32 LU without pivoting at n = 12288 (764 workgroups in the first panel launch, dispatched over
+    time: the diagonal block must not be overwritten while later workgroups still read it),
+    refined in FP64 on a diagonally dominant system: converges to the FP64 LU solve."""
+    n = 12288
+    rng = np.random.default_rng(123)
+    a = rng.standard_normal((n, n)) + 4.0 * np.sqrt(n) * np.eye(n)
+    b = rng.standard_normal((n, 1))
+    dA = DeviceArray.from_numpy(a)
+    dB = DeviceArray.from_numpy(b)
+    lu32 = LA.LU(n, LA.FP32, pivoting=False).factor(dA)
+    x = DeviceArray((n, 1))
+    _, iters, _, relres = LA.solve_refine(dA, dB, lu32, tol=1e-13, out=x)
+    assert float(np.max(relres)) <= 1e-13 and iters <= 6
+    lu64 = LA.LU(n, LA.FP64, pivoting="auto").factor(dA)
+    assert np.array_equal(lu64.pivots(), np.arange(n))
+    x64 = lu64.solve(b)
+    assert _rel(x.numpy(), x64) < 1e-11
