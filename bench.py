@@ -122,7 +122,8 @@ def roofline_from(prof) -> dict:
         "note": "FP64 vector-ALU bound (MI355X FP64 vector peak = FP64 MFMA peak = 78.6 TFLOP/s); "
                 f"algorithmic {FLOP_PER_PAIR + FLOP_WEIGHT:.0f} FLOP per (segment, point) pair, "
                 f"{pairs / max(cnt, 1):.4g} pairs per launch, avg {avg_ms:.3f} ms over {cnt} launches "
-                "(HIP events on the library stream around this kernel only, over the timed region)",
+                "(HIP events on the side stream the kernel is launched on, around this kernel only, over "
+                "the timed region; it runs beside the bound solve, capped at 5 workgroups per CU)",
     }
 
 
