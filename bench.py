@@ -40,7 +40,7 @@ PEAK_FP64_TFLOPS = 78.6  # MI355X FP64 (vector and MFMA)
 # WRITE_SIZE in two runs of this script (--copy-calibration), each scaled by what the same
 # counter reads for the script's own 1 GiB copy (FETCH x 2.0, WRITE x 1.0); the commands and
 # numbers are in profiles/wake_self_velocity_c3_fp64_traffic.md.
-SELF_VELOCITY_HBM_BYTES = 80.27e6
+SELF_VELOCITY_HBM_BYTES = 83.27e6
 PEAK_HBM_GBS = 8000.0
 FLOP_PER_PAIR = 87.0  # induction pair (src/core/flow_solver.c:114-135, atan2 counted as 20)
 FLOP_WEIGHT = 6.0  # Gamma weighting of a point-velocity sum
