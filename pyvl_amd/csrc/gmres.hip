@@ -374,7 +374,8 @@ int pvl_gmres_refine(int n, const double *A64_dev, const float *A32_dev, const d
     diag_inv_kernel<<<(n + 255) / 256, 256, 0, stream()>>>(A32_dev, n, g_gw.dinv);
     PVL_HIP(hipGetLastError());
     // norms[0] = max|b|, norms[2] = max|r| of the current outer step
-    PVL_HIP(hipMemsetAsync(g_gw.norms, 0, 4 * sizeof(double), stream()));
+    if (int e = device_fill32(g_gw.norms, 0u, 4 * sizeof(double)))
+        return e;
     maxabs_kernel<<<64, 256, 0, stream()>>>(b_dev, n, g_gw.norms);
     PVL_HIP(hipGetLastError());
     int total_steps = 0, outer = 0, pending_cycle = 0;
@@ -384,7 +385,8 @@ int pvl_gmres_refine(int n, const double *A64_dev, const float *A32_dev, const d
     {
         if (int st = pvl_residual(n, A64_dev, x_dev, b_dev, g_gw.r64, 1))
             return st;
-        PVL_HIP(hipMemsetAsync(g_gw.norms + 2, 0, sizeof(double), stream()));
+        if (int e = device_fill32(g_gw.norms + 2, 0u, sizeof(double)))
+            return e;
         maxabs_kernel<<<64, 256, 0, stream()>>>(g_gw.r64, n, g_gw.norms + 2);
         PVL_HIP(hipGetLastError());
         // one host synchronisation per outer step: residual norm and the last cycle's result
@@ -409,7 +411,8 @@ int pvl_gmres_refine(int n, const double *A64_dev, const float *A32_dev, const d
         if (bmax == 0.0)
         {
             // b = 0: the solution is x = 0
-            PVL_HIP(hipMemsetAsync(x_dev, 0, sizeof(double) * n, stream()));
+            if (int e = device_fill32(x_dev, 0u, sizeof(double) * n))
+                return e;
             rel0 = rel = 0.0;
             converged = true;
             break;

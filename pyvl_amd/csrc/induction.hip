@@ -574,7 +574,8 @@ int point_velocity_impl(const pvl_mesh *m, double tol, const double *pos, const 
     if (n_active == 0)
     {
         if (!accumulate)
-            PVL_HIP(hipMemsetAsync(out, 0, sizeof(double) * 3 * n_pts, stream()));
+            if (int st = device_fill32(out, 0u, sizeof(double) * 3 * n_pts))
+                return st;
         return PVL_OK;
     }
     const unsigned n_chunks = point_velocity_chunks(n_pts, n_active, prof_name);

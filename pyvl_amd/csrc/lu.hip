@@ -982,6 +982,39 @@ struct Workspace
                         // [3]: an interchange was due, [4]: workgroups done (unpivoted panel)
 };
 Workspace g_ws;
+// Pinned, coherent host copy of err[0..3], written by a kernel: the end-of-factorisation
+// readback needs no blit (the runtime's copy kernels can wait for CU room behind the side
+// stream's work).
+volatile int *g_status_host = nullptr;
+int *g_status_host_dev = nullptr;
+
+__global__ void factor_reset_kernel(unsigned *__restrict__ x0, int *__restrict__ status)
+{
+    main_stream_priority();
+    if (threadIdx.x == 0)
+    {
+        *x0 = 0u;             // panel exchange sequence
+        status[0] = 0;        // first zero pivot
+        status[1] = 0;        // a bounded wait timed out
+        status[2] = 0x7F7F7F7F; // no-pivot zero pivots: min
+        status[3] = 0;        // an interchange was due
+    }
+}
+
+__global__ void status_to_host_kernel(const int *__restrict__ status, int *__restrict__ host)
+{
+    main_stream_priority();
+    if (threadIdx.x < 4)
+        host[threadIdx.x] = status[threadIdx.x];
+}
+
+__global__ void identity_kernel(int *__restrict__ perm, int n)
+{
+    main_stream_priority();
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i < n)
+        perm[i] = i;
+}
 
 int ensure_ws()
 {
@@ -990,6 +1023,10 @@ int ensure_ws()
         PVL_HIP(hipMalloc(&g_ws.panelx, sizeof(PanelX<double>)));
         PVL_HIP(hipMalloc(&g_ws.err, 8 * sizeof(int)));
         PVL_HIP(hipMemset(g_ws.err, 0, 8 * sizeof(int)));
+        void *h = nullptr;
+        PVL_HIP(hipHostMalloc(&h, 16 * sizeof(int), hipHostMallocCoherent | hipHostMallocMapped));
+        g_status_host = (volatile int *)h;
+        PVL_HIP(hipHostGetDevicePointer((void **)&g_status_host_dev, h, 0));
     }
     return PVL_OK;
 }
@@ -1011,13 +1048,11 @@ template <typename T> int factor_impl(int n, T *A, int *piv, int *info_out, bool
     }
     PanelX<T> *X = (PanelX<T> *)g_ws.panelx;
     int *status = g_ws.err;
-    PVL_HIP(hipMemsetAsync(X, 0, sizeof(unsigned), stream()));
-    PVL_HIP(hipMemsetAsync(status, 0, 2 * sizeof(int), stream()));
+    factor_reset_kernel<<<1, 64, 0, stream()>>>((unsigned *)X, status);
+    PVL_HIP(hipGetLastError());
     unsigned base = 0;
     // Panel width: 32 up to n = 2048, where single-workgroup register panels cover every
     // panel and the trailing updates are latency-bound anyway; 64 above.
-    PVL_HIP(hipMemsetAsync(status + 2, 0x7F, sizeof(int), stream())); // no-pivot zero pivots: min
-    PVL_HIP(hipMemsetAsync(status + 3, 0, sizeof(int), stream()));    // an interchange was due
     const int nbf = !pivoting ? NB : (n <= 2048 ? kPanelWgCols : NB);
     for (int k = 0; k < n; k += nbf)
     {
@@ -1087,9 +1122,12 @@ template <typename T> int factor_impl(int n, T *A, int *piv, int *info_out, bool
             PVL_HIP(hipGetLastError());
         }
     }
-    int st2[4];
-    PVL_HIP(hipMemcpyAsync(st2, status, sizeof(st2), hipMemcpyDeviceToHost, stream()));
+    status_to_host_kernel<<<1, 64, 0, stream()>>>(status, g_status_host_dev);
+    PVL_HIP(hipGetLastError());
     PVL_HIP(hipStreamSynchronize(stream()));
+    int st2[4];
+    for (int i = 0; i < 4; ++i)
+        st2[i] = g_status_host[i];
     if (st2[1])
     {
         set_error("LU panel barrier timed out");
@@ -1119,9 +1157,15 @@ struct pvl_lu
 namespace
 {
 
-template <typename T> int finish_factor(pvl_lu *h)
+template <typename T> int finish_factor(pvl_lu *h, bool interchanged)
 {
     const int n = h->n;
+    if (!interchanged)
+    {
+        identity_kernel<<<(n + 255) / 256, 256, 0, stream()>>>(h->perm, n);
+        PVL_HIP(hipGetLastError());
+        return PVL_OK;
+    }
     // permutation from the interchange list (LAPACK order), computed on the host
     std::vector<int> piv(n), perm(n);
     PVL_HIP(hipMemcpyAsync(piv.data(), h->piv, sizeof(int) * n, hipMemcpyDeviceToHost, stream()));
@@ -1175,7 +1219,8 @@ template <typename T> int solve_impl(pvl_lu *h, T *B, int nrhs)
                 trsv_step_kernel<T, false><<<st < 0 ? 1 : nblk - 1 - st, 256, 0, stream()>>>(Ap, n, n, work, nq, st);
             PVL_HIP(hipGetLastError());
         }
-        PVL_HIP(hipMemcpyAsync(B, work, sizeof(T) * (size_t)n * nq, hipMemcpyDeviceToDevice, stream()));
+        if (int e = device_copy(B, work, sizeof(T) * (size_t)n * nq))
+            return e;
     }
     return PVL_OK;
 }
@@ -1267,13 +1312,16 @@ int pvl_lu_factor(pvl_lu *h, const double *A_dev, int *info_out)
     // another row at any column, factor again with partial pivoting.  Either way the
     // result is partial pivoting's factorisation.
     const bool try_free = h->pivoting != PVL_PIVOT_PARTIAL;
+    bool pivoted = false;
     for (int attempt = 0; attempt < 2; ++attempt)
     {
         const bool pivot = attempt == 1 || !try_free;
+        pivoted = pivot;
         bool would_pivot = false;
         if (h->prec == PVL_FP64)
         {
-            PVL_HIP(hipMemcpyAsync(h->lu, A_dev, sizeof(double) * cnt, hipMemcpyDeviceToDevice, stream()));
+            if (int e = device_copy(h->lu, A_dev, sizeof(double) * cnt))
+                return e;
             st = factor_impl<double>(n, (double *)h->lu, h->piv, &info, pivot, &would_pivot);
         }
         else
@@ -1286,7 +1334,7 @@ int pvl_lu_factor(pvl_lu *h, const double *A_dev, int *info_out)
             break;
     }
     if (!st)
-        st = h->prec == PVL_FP64 ? finish_factor<double>(h) : finish_factor<float>(h);
+        st = h->prec == PVL_FP64 ? finish_factor<double>(h, pivoted) : finish_factor<float>(h, pivoted);
     if (info_out)
         *info_out = info;
     if (!st && info)
@@ -1372,7 +1420,8 @@ int pvl_solve_refine(pvl_lu *h, const double *A_dev, const double *B_dev, double
     if (!R || !D32 || !norms)
         return PVL_ERR_NOMEM;
     ProfScope prof_all("solve_refine");
-    PVL_HIP(hipMemsetAsync(X_dev, 0, sizeof(double) * cnt, stream()));
+    if (int e = device_fill32(X_dev, 0u, sizeof(double) * cnt))
+        return e;
     // norms[nrhs + q] = max|b_q| (once); norms[q] = max|r_q| of the current iterate
     colabsmax_kernel<<<nrhs, 256, 0, stream()>>>(B_dev, n, nrhs, norms + nrhs);
     PVL_HIP(hipGetLastError());

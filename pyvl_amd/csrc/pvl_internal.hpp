@@ -51,6 +51,11 @@ int check_hip(hipError_t e, const char *what);
 // Scratch device buffer reused across calls, grown on demand (one per slot).
 void *scratch(int slot, size_t bytes);
 
+// Device-to-device copy / 32-bit-word fill on stream s (nullptr: the library stream) with the
+// library's own kernels (device_fill32 needs 4-byte aligned whole words).
+int device_copy(void *dst, const void *src, size_t bytes, hipStream_t s = nullptr);
+int device_fill32(void *dst, uint32_t value, size_t bytes, hipStream_t s = nullptr);
+
 } // namespace pvl
 
 struct pvl_mesh;

@@ -113,6 +113,65 @@ hipStream_t side_stream()
 }
 
 // ---------------------------------------------------------------------------------------
+// Device-to-device copies and fills as the library's own small kernels (at the library
+// stream's priority, a few VGPRs): the runtime's blit kernels can wait for CU room while the
+// side stream's kernel fills the device.
+__global__ void __launch_bounds__(256) copy_words_kernel(uint32_t *__restrict__ dst, const uint32_t *__restrict__ src,
+                                                         size_t n)
+{
+    main_stream_priority();
+    for (size_t i = blockIdx.x * (size_t)256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256)
+        dst[i] = src[i];
+}
+__global__ void __launch_bounds__(256) copy_bytes_kernel(uint8_t *__restrict__ dst, const uint8_t *__restrict__ src,
+                                                         size_t n)
+{
+    main_stream_priority();
+    for (size_t i = blockIdx.x * (size_t)256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256)
+        dst[i] = src[i];
+}
+__global__ void __launch_bounds__(256) fill_words_kernel(uint32_t *__restrict__ dst, uint32_t v, size_t n)
+{
+    main_stream_priority();
+    for (size_t i = blockIdx.x * (size_t)256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256)
+        dst[i] = v;
+}
+
+static unsigned blocks_for(size_t n)
+{
+    const size_t b = (n + 255) / 256;
+    return (unsigned)(b < 1024 ? (b ? b : 1) : 1024);
+}
+
+int device_copy(void *dst, const void *src, size_t bytes, hipStream_t s)
+{
+    if (bytes == 0)
+        return PVL_OK;
+    if (!s)
+        s = g_stream;
+    if (((uintptr_t)dst | (uintptr_t)src | bytes) % 4 == 0)
+        copy_words_kernel<<<blocks_for(bytes / 4), 256, 0, s>>>((uint32_t *)dst, (const uint32_t *)src, bytes / 4);
+    else
+        copy_bytes_kernel<<<blocks_for(bytes), 256, 0, s>>>((uint8_t *)dst, (const uint8_t *)src, bytes);
+    PVL_HIP(hipGetLastError());
+    return PVL_OK;
+}
+
+int device_fill32(void *dst, uint32_t value, size_t bytes, hipStream_t s)
+{
+    if (bytes == 0)
+        return PVL_OK;
+    if (((uintptr_t)dst | bytes) % 4 != 0)
+    {
+        set_error("device_fill32: %zu bytes at %p are not whole aligned 32-bit words", bytes, dst);
+        return PVL_ERR_ARG;
+    }
+    fill_words_kernel<<<blocks_for(bytes / 4), 256, 0, s ? s : g_stream>>>((uint32_t *)dst, value, bytes / 4);
+    PVL_HIP(hipGetLastError());
+    return PVL_OK;
+}
+
+// ---------------------------------------------------------------------------------------
 // Scratch buffers: grown on demand, never shrunk; freed at process end by the runtime.
 static std::vector<std::pair<void *, size_t>> g_scratch(16, {nullptr, 0});
 
@@ -312,10 +371,7 @@ int pvl_memcpy_d2d(void *dst, const void *src, size_t bytes)
 {
     if (int st = ensure_init())
         return st;
-    if (bytes == 0)
-        return PVL_OK;
-    PVL_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, g_stream));
-    return PVL_OK;
+    return device_copy(dst, src, bytes, g_stream);
 }
 
 int pvl_memset_zero(void *dst, size_t bytes)
@@ -324,6 +380,8 @@ int pvl_memset_zero(void *dst, size_t bytes)
         return st;
     if (bytes == 0)
         return PVL_OK;
+    if (((uintptr_t)dst | bytes) % 4 == 0)
+        return device_fill32(dst, 0u, bytes, g_stream);
     PVL_HIP(hipMemsetAsync(dst, 0, bytes, g_stream));
     return PVL_OK;
 }

@@ -458,7 +458,8 @@ int pvl_wake_update(pvl_wake *w, double time, const pvl_mesh *bound, const doubl
     }
     const unsigned cr = w->step_count < w->R ? w->step_count : w->R;
     const unsigned n_old = cr * n_te;
-    PVL_HIP(hipMemcpyAsync(w->d_old, w->d_upos, sizeof(double) * 3 * n_old, hipMemcpyDeviceToDevice, stream()));
+    if (int st = device_copy(w->d_old, w->d_upos, sizeof(double) * 3 * n_old))
+        return st;
     {
         if (int st = point_velocity_impl(bound, w->tol, bound_pos_dev, circulation_dev, n_old, w->d_old, w->d_vel, 0,
                                          precision, bound->n_lines, "wake_bound_velocity"))
@@ -538,8 +539,10 @@ int pvl_wake_apply_corrections(pvl_wake *w, unsigned n_cpts, const double *cpts_
     if (na == 0)
         return PVL_OK;
     const size_t nc = (size_t)w->R * w->n_shed;
-    PVL_HIP(hipMemcpyAsync(w->d_circ_x, w->d_circ[w->cur], sizeof(double) * nc, hipMemcpyDeviceToDevice, stream()));
-    PVL_HIP(hipMemsetAsync(w->d_circ_x, 0, sizeof(double) * w->n_shed, stream()));
+    if (int st = device_copy(w->d_circ_x, w->d_circ[w->cur], sizeof(double) * nc))
+        return st;
+    if (int st = device_fill32(w->d_circ_x, 0u, sizeof(double) * w->n_shed))
+        return st;
     double *v = (double *)scratch(13, sizeof(double) * 3 * (size_t)n_cpts);
     if (!v)
         return PVL_ERR_NOMEM;
@@ -563,7 +566,8 @@ int pvl_wake_get_velocity(pvl_wake *w, unsigned n_pts, const double *pts_dev, do
     if (na == 0)
     {
         if (!accumulate && n_pts)
-            PVL_HIP(hipMemsetAsync(out_dev, 0, sizeof(double) * 3 * n_pts, stream()));
+            if (int st = device_fill32(out_dev, 0u, sizeof(double) * 3 * n_pts))
+                return st;
         return PVL_OK;
     }
     return point_velocity_impl(w->full, w->tol, w->d_upos, w->d_circ[w->cur], n_pts, pts_dev, out_dev, accumulate,
