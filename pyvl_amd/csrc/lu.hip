@@ -615,7 +615,10 @@ __global__ void __launch_bounds__(256) trsm_kernel(T *__restrict__ A, int lda, i
 
 // A22 -= L21 U12 on MFMA.  64x64 output tile per workgroup, 32x32 per wave (2x2 MFMA
 // 16x16 tiles).  Operand lane layout of the 16x16x4 MFMAs: lane l supplies A[l%16][l/16]
-// and B[l/16][l%16]; the accumulator row of element v is Mfma<T>::out_row.
+// and B[l/16][l%16]; the accumulator row of element v is Mfma<T>::out_row.  The products are
+// summed from zero and subtracted from C once (as dgemm's C - A B: one rounding at |C|'s
+// magnitude, not one per k on diagonally dominant updates); the C tile is loaded into
+// registers at the start, while the operand tiles are staged in LDS.
 template <typename T>
 __global__ void __launch_bounds__(256) gemm_kernel(T *__restrict__ A, int lda, int m2, int n2, int k, int jb)
 {
@@ -627,6 +630,28 @@ __global__ void __launch_bounds__(256) gemm_kernel(T *__restrict__ A, int lda, i
     const int i0 = blockIdx.y * BM, j0 = blockIdx.x * BN;
     const int rowA = k + jb, colB = k + jb;
     const int kpad = (jb + 3) & ~3;
+    using acc_t = typename Mfma<T>::acc_t;
+    const int w = tid >> 6, lane = tid & 63;
+    const int wm = w >> 1, wn = w & 1;
+    const int lr = lane & 15, lk = lane >> 4;
+    T cprev[2][2][4];
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b)
+#pragma unroll
+            for (int v = 0; v < 4; ++v)
+            {
+                const int r = i0 + 32 * wm + 16 * a + Mfma<T>::out_row(lk, v);
+                const int c = j0 + 32 * wn + 16 * b + lr;
+                cprev[a][b][v] = (r < m2 && c < n2) ? A[(size_t)(rowA + r) * lda + colB + c] : T(0);
+            }
+    acc_t acc[2][2];
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b)
+            acc[a][b] = acc_t{0, 0, 0, 0};
     for (int idx = tid; idx < BM * NB; idx += 256)
     {
         const int r = idx / NB, t = idx % NB;
@@ -638,16 +663,6 @@ __global__ void __launch_bounds__(256) gemm_kernel(T *__restrict__ A, int lda, i
         Bs[t][c] = (t < jb && j0 + c < n2) ? A[(size_t)(k + t) * lda + colB + j0 + c] : T(0);
     }
     __syncthreads();
-    using acc_t = typename Mfma<T>::acc_t;
-    const int w = tid >> 6, lane = tid & 63;
-    const int wm = w >> 1, wn = w & 1;
-    const int lr = lane & 15, lk = lane >> 4;
-    acc_t acc[2][2];
-#pragma unroll
-    for (int a = 0; a < 2; ++a)
-#pragma unroll
-        for (int b = 0; b < 2; ++b)
-            acc[a][b] = acc_t{0, 0, 0, 0};
     for (int kk = 0; kk < kpad; kk += 4)
     {
         T af[2], bf[2];
@@ -673,10 +688,7 @@ __global__ void __launch_bounds__(256) gemm_kernel(T *__restrict__ A, int lda, i
                 const int r = i0 + 32 * wm + 16 * a + Mfma<T>::out_row(lk, v);
                 const int c = j0 + 32 * wn + 16 * b + lr;
                 if (r < m2 && c < n2)
-                {
-                    T *dst = A + (size_t)(rowA + r) * lda + colB + c;
-                    *dst = *dst - acc[a][b][v];
-                }
+                    A[(size_t)(rowA + r) * lda + colB + c] = cprev[a][b][v] - acc[a][b][v];
             }
 }
 
