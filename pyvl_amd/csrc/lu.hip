@@ -556,14 +556,14 @@ __global__ void __launch_bounds__(256) laswp_kernel(T *__restrict__ A, int lda, 
     }
 }
 
-// U12 = L11^-1 A12 for the columns right of the panel.  Each wave covers 16 columns with 4
+// U12 = L11^-1 A12 for columns [c0, c1) right of the panel.  Each wave covers 16 columns with 4
 // lanes per column: lane p of a quad holds rows p, p + 4, .. of its column in registers.  Step
 // t broadcasts the final x_t from its owner lane over the quad (DPP) and every lane subtracts
 // l_it x_t from its rows i > t, with the l_it each lane needs contiguous in LDS
 // (Lq[t][p * KQ + k] = l_(4k+p),t).  Per element the operations are those of dtrsm's column
 // sweep.  No barrier after the load.  JB = panel width (32 or 64).
 template <typename T, int JB>
-__global__ void __launch_bounds__(256) trsm_kernel(T *__restrict__ A, int lda, int n, int k)
+__global__ void __launch_bounds__(256) trsm_kernel(T *__restrict__ A, int lda, int c0, int c1, int k)
 {
     main_stream_priority();
     constexpr int KQ = JB / 4;
@@ -575,10 +575,10 @@ __global__ void __launch_bounds__(256) trsm_kernel(T *__restrict__ A, int lda, i
     }
     __syncthreads();
     const int lane = threadIdx.x & 63, p = lane & 3;
-    const int col = k + JB + (blockIdx.x * 4 + (threadIdx.x >> 6)) * 16 + (lane >> 2);
-    if (k + JB + (blockIdx.x * 4 + (threadIdx.x >> 6)) * 16 >= n)
+    const int col = c0 + (blockIdx.x * 4 + (threadIdx.x >> 6)) * 16 + (lane >> 2);
+    if (c0 + (blockIdx.x * 4 + (threadIdx.x >> 6)) * 16 >= c1)
         return;
-    const bool live = col < n;
+    const bool live = col < c1;
     T x[KQ];
 #pragma unroll
     for (int q = 0; q < KQ; ++q)
@@ -615,21 +615,35 @@ __global__ void __launch_bounds__(256) trsm_kernel(T *__restrict__ A, int lda, i
 
 // A22 -= L21 U12 on MFMA.  64x64 output tile per workgroup, 32x32 per wave (2x2 MFMA
 // 16x16 tiles).  Operand lane layout of the 16x16x4 MFMAs: lane l supplies A[l%16][l/16]
-// and B[l/16][l%16]; the accumulator row of element v is Mfma<T>::out_row.  The products are
-// summed from zero and subtracted from C once (as dgemm's C - A B: one rounding at |C|'s
-// magnitude, not one per k on diagonally dominant updates); the C tile is loaded into
-// registers at the start, while the operand tiles are staged in LDS.
+// and B[l/16][l%16]; the accumulator row of element v is Mfma<T>::out_row.  Each 64-wide K
+// segment (one panel's L21 U12) is summed from zero and subtracted from C once (as dgemm's
+// C - A B: one rounding at |C|'s magnitude per panel, not one per k on diagonally dominant
+// updates); the C tile stays in registers from the start to the end, across segments.  K is staged through LDS in chunks of 32 (33 KB for FP64: four
+// workgroups per CU keep the matrix cores fed while others wait on memory).
+// Tile order (1-D grid): consecutive workgroups go round-robin to the 8 XCDs, each with its
+// own L2, so workgroup g works for XCD g % 8, which owns a stripe of cpx column tiles and
+// sweeps it row tile by row tile: its U12 stripe stays in its L2 and each L21 tile is
+// reused across the stripe while hot (rather than every XCD streaming all of U12 per row).
+constexpr int kXcds = 8;
+// C = rows [row0, row0 + m2) x columns [col0, col0 + n2) of A; the update subtracts
+// A[C rows, kk0 .. kk0 + K) times A[kk0 .. kk0 + K, C columns].
 template <typename T>
-__global__ void __launch_bounds__(256) gemm_kernel(T *__restrict__ A, int lda, int m2, int n2, int k, int jb)
+__global__ void __launch_bounds__(256) gemm_kernel(T *__restrict__ A, int lda, int row0, int m2, int col0, int n2,
+                                                   int kk0, int K, int cpx)
 {
     main_stream_priority<2>();
-    constexpr int BM = 64, BN = 64;
-    __shared__ T As[BM][NB + 1];
-    __shared__ T Bs[NB][BN + 1];
+    constexpr int BM = 64, BN = 64, KC = 32;
+    // Row strides 36 and 80 elements: the operand reads (16 rows x 4 k, 4 k x 16 columns per
+    // wave) then fall on distinct LDS banks apart from the two passes a 64-lane read needs.
+    __shared__ T As[BM][KC + 4];
+    __shared__ T Bs[KC][BN + 16];
     const int tid = threadIdx.x;
-    const int i0 = blockIdx.y * BM, j0 = blockIdx.x * BN;
-    const int rowA = k + jb, colB = k + jb;
-    const int kpad = (jb + 3) & ~3;
+    const int xcd = blockIdx.x % kXcds, li = blockIdx.x / kXcds;
+    const int tr = li / cpx, tc = xcd * cpx + li % cpx;
+    if (tr * BM >= m2 || tc * BN >= n2)
+        return; // the whole workgroup: outside C
+    const int i0 = tr * BM, j0 = tc * BN;
+    const int rowA = row0, colB = col0, k = kk0, jb = K;
     using acc_t = typename Mfma<T>::acc_t;
     const int w = tid >> 6, lane = tid & 63;
     const int wm = w >> 1, wn = w & 1;
@@ -652,31 +666,52 @@ __global__ void __launch_bounds__(256) gemm_kernel(T *__restrict__ A, int lda, i
 #pragma unroll
         for (int b = 0; b < 2; ++b)
             acc[a][b] = acc_t{0, 0, 0, 0};
-    for (int idx = tid; idx < BM * NB; idx += 256)
+    for (int k0 = 0; k0 < jb; k0 += KC)
     {
-        const int r = idx / NB, t = idx % NB;
-        As[r][t] = (i0 + r < m2 && t < jb) ? A[(size_t)(rowA + i0 + r) * lda + k + t] : T(0);
-    }
-    for (int idx = tid; idx < NB * BN; idx += 256)
-    {
-        const int t = idx / BN, c = idx % BN;
-        Bs[t][c] = (t < jb && j0 + c < n2) ? A[(size_t)(k + t) * lda + colB + j0 + c] : T(0);
-    }
-    __syncthreads();
-    for (int kk = 0; kk < kpad; kk += 4)
-    {
-        T af[2], bf[2];
+        if (k0)
+            __syncthreads(); // the previous chunk's operands are consumed
+        for (int idx = tid; idx < BM * KC; idx += 256)
+        {
+            const int r = idx / KC, t = idx % KC;
+            As[r][t] = (i0 + r < m2 && k0 + t < jb) ? A[(size_t)(rowA + i0 + r) * lda + k + k0 + t] : T(0);
+        }
+        for (int idx = tid; idx < KC * BN; idx += 256)
+        {
+            const int t = idx / BN, c = idx % BN;
+            Bs[t][c] = (k0 + t < jb && j0 + c < n2) ? A[(size_t)(k + k0 + t) * lda + colB + j0 + c] : T(0);
+        }
+        __syncthreads();
+        const int kend = min(KC, (jb - k0 + 3) & ~3);
+        for (int kk = 0; kk < kend; kk += 4)
+        {
+            T af[2], bf[2];
 #pragma unroll
-        for (int a = 0; a < 2; ++a)
-            af[a] = As[32 * wm + 16 * a + lr][kk + lk];
-#pragma unroll
-        for (int b = 0; b < 2; ++b)
-            bf[b] = Bs[kk + lk][32 * wn + 16 * b + lr];
-#pragma unroll
-        for (int a = 0; a < 2; ++a)
+            for (int a = 0; a < 2; ++a)
+                af[a] = As[32 * wm + 16 * a + lr][kk + lk];
 #pragma unroll
             for (int b = 0; b < 2; ++b)
-                acc[a][b] = Mfma<T>::mma(af[a], bf[b], acc[a][b]);
+                bf[b] = Bs[kk + lk][32 * wn + 16 * b + lr];
+#pragma unroll
+            for (int a = 0; a < 2; ++a)
+#pragma unroll
+                for (int b = 0; b < 2; ++b)
+                    acc[a][b] = Mfma<T>::mma(af[a], bf[b], acc[a][b]);
+        }
+        if ((k0 + KC) % NB == 0 || k0 + KC >= jb)
+        {
+            // end of a 64-wide K segment (one panel): subtract its sum, as a per-panel update
+            // would, so the result does not depend on how many panels one launch covers
+#pragma unroll
+            for (int a = 0; a < 2; ++a)
+#pragma unroll
+                for (int b = 0; b < 2; ++b)
+                {
+#pragma unroll
+                    for (int v = 0; v < 4; ++v)
+                        cprev[a][b][v] = cprev[a][b][v] - acc[a][b][v];
+                    acc[a][b] = acc_t{0, 0, 0, 0};
+                }
+        }
     }
 #pragma unroll
     for (int a = 0; a < 2; ++a)
@@ -688,7 +723,7 @@ __global__ void __launch_bounds__(256) gemm_kernel(T *__restrict__ A, int lda, i
                 const int r = i0 + 32 * wm + 16 * a + Mfma<T>::out_row(lk, v);
                 const int c = j0 + 32 * wn + 16 * b + lr;
                 if (r < m2 && c < n2)
-                    A[(size_t)(rowA + r) * lda + colB + c] = cprev[a][b][v] - acc[a][b][v];
+                    A[(size_t)(rowA + r) * lda + colB + c] = cprev[a][b][v];
             }
 }
 
@@ -1066,7 +1101,40 @@ template <typename T> int factor_impl(int n, T *A, int *piv, int *info_out, bool
     // Panel width: 32 up to n = 2048, where single-workgroup register panels cover every
     // panel and the trailing updates are latency-bound anyway; 64 above.
     const int nbf = !pivoting ? NB : (n <= 2048 ? kPanelWgCols : NB);
-    for (int k = 0; k < n; k += nbf)
+    // Outer blocks of 4 panels for n > 2048: the trailing matrix right of and below an outer
+    // block is updated in one launch covering K = 256 (a quarter of the C traffic of four
+    // K = 64 updates; the arithmetic is that of four per-panel updates);
+    // inside the block each panel updates the rest of the block row (all columns to the
+    // right) and the block's remaining columns below it.  For n <= 2048 the outer block is
+    // one panel and the updates reduce to the usual one per panel.
+    const int nbo = n > 2048 ? 4 * nbf : nbf;
+    auto gemm = [&](int row0, int m2, int col0, int n2, int kk0, int K) -> int {
+        if (m2 <= 0 || n2 <= 0 || K <= 0)
+            return PVL_OK;
+        const int nt = (n2 + 63) / 64, cpx = (nt + kXcds - 1) / kXcds;
+        const int mt = (m2 + 63) / 64;
+        ProfScope prof(sizeof(T) == 8 ? "lu_gemm_f64" : "lu_gemm_f32");
+        gemm_kernel<T><<<kXcds * mt * cpx, 256, 0, stream()>>>(A, n, row0, m2, col0, n2, kk0, K, cpx);
+        PVL_HIP(hipGetLastError());
+        return PVL_OK;
+    };
+    // U12 of the panel at k for columns [c0, c1)
+    auto trsm = [&](int k, int c0, int c1) -> int {
+        if (c1 <= c0)
+            return PVL_OK;
+        const int jb = std::min(nbf, n - k);
+        ProfScope prof(sizeof(T) == 8 ? "lu_trsm_f64" : "lu_trsm_f32");
+        if (jb == NB)
+            trsm_kernel<T, NB><<<(c1 - c0 + 63) / 64, 256, 0, stream()>>>(A, n, c0, c1, k);
+        else
+            trsm_kernel<T, kPanelWgCols><<<(c1 - c0 + 63) / 64, 256, 0, stream()>>>(A, n, c0, c1, k);
+        PVL_HIP(hipGetLastError());
+        return PVL_OK;
+    };
+    for (int ko = 0; ko < n; ko += nbo)
+    {
+    const int kend = std::min(n, ko + nbo);
+    for (int k = ko; k < kend; k += nbf)
     {
         const int jb = std::min(nbf, n - k);
         const int m = n - k;
@@ -1116,23 +1184,41 @@ template <typename T> int factor_impl(int n, T *A, int *piv, int *info_out, bool
             laswp_kernel<T><<<(n + 255) / 256, 256, 0, stream()>>>(A, n, n, k, jb, piv);
             PVL_HIP(hipGetLastError());
         }
-        const int n2 = n - k - jb;
-        if (n2 > 0)
+        if (pivoting)
         {
-            if (pivoting) // the unpivoted panel kernel computes U12 itself
-            {
-                ProfScope prof(sizeof(T) == 8 ? "lu_trsm_f64" : "lu_trsm_f32");
-                if (jb == NB)
-                    trsm_kernel<T, NB><<<(n2 + 63) / 64, 256, 0, stream()>>>(A, n, n, k);
-                else
-                    trsm_kernel<T, kPanelWgCols><<<(n2 + 63) / 64, 256, 0, stream()>>>(A, n, n, k);
-                PVL_HIP(hipGetLastError());
-            }
-            const dim3 grid((n2 + 63) / 64, (n2 + 63) / 64);
-            ProfScope prof(sizeof(T) == 8 ? "lu_gemm_f64" : "lu_gemm_f32");
-            gemm_kernel<T><<<grid, 256, 0, stream()>>>(A, n, n2, n2, k, jb);
-            PVL_HIP(hipGetLastError());
+            // U12 and the update within the outer block's columns only: the block row right
+            // of the outer block is formed after all of the block's interchanges (as blocked
+            // dgetrf does), since a later panel's swap may bring up a row from below
+            if (int st = trsm(k, k + jb, kend))
+                return st;
+            if (int st = gemm(k + jb, n - (k + jb), k + jb, kend - (k + jb), k, jb))
+                return st;
         }
+        else if (n - k - jb > 0)
+        {
+            // no interchanges: the panel kernel formed U12 for every column to the right, so
+            // the rest of the block row is updated across all columns right away
+            if (int st = gemm(k + jb, kend - (k + jb), k + jb, n - (k + jb), k, jb))
+                return st;
+            if (int st = gemm(kend, n - kend, k + jb, kend - (k + jb), k, jb))
+                return st;
+        }
+    }
+    if (pivoting)
+    {
+        // block row right of the outer block: U12 = L11^-1 A12 panel by panel
+        for (int k = ko; k < kend; k += nbf)
+        {
+            const int jb = std::min(nbf, n - k);
+            if (int st = trsm(k, kend, n))
+                return st;
+            if (int st = gemm(k + jb, kend - (k + jb), kend, n - kend, k, jb))
+                return st;
+        }
+    }
+    // the trailing matrix, once per outer block
+    if (int st = gemm(kend, n - kend, kend, n - kend, ko, kend - ko))
+        return st;
     }
     status_to_host_kernel<<<1, 64, 0, stream()>>>(status, g_status_host_dev);
     PVL_HIP(hipGetLastError());
