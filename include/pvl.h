@@ -14,9 +14,10 @@
  *     (the reference's geo_id_t / line_t, src/core/common.h:46-56, src/core/mesh.h:10-14).
  *   - Every function returns an int status (PVL_OK = 0, negative on failure) and sets a
  *     thread-local message readable with pvl_last_error().
- *   - All device work is issued on one library stream per process (default priority);
- *     functions that return device results are asynchronous unless stated otherwise,
- *     and pvl_synchronize() waits for them.
+ *   - All device work is issued on one library stream per process (default priority),
+ *     except the work pvl_wake_begin_update starts on a second library stream (also default
+ *     priority, ordered with events); functions that return device results are
+ *     asynchronous unless stated otherwise, and pvl_synchronize() waits for both streams.
  */
 #ifndef PVL_H
 #define PVL_H

@@ -320,6 +320,8 @@ int pvl_synchronize(void)
     if (int st = ensure_init())
         return st;
     PVL_HIP(hipStreamSynchronize(g_stream));
+    if (g_side)
+        PVL_HIP(hipStreamSynchronize(g_side));
     return PVL_OK;
 }
 
