@@ -940,43 +940,74 @@ __global__ void __launch_bounds__(kTrsvWgThreads) __attribute__((amdgpu_waves_pe
         B[i] = x[i];
 }
 
-// R = B - A X  (FP64; A n x n row-major; X, B, R n x nrhs).  One wave per row.
-template <typename TA>
-__global__ void __launch_bounds__(256) residual_kernel(const TA *__restrict__ A, int lda, int n,
-                                                       const double *__restrict__ X, const double *__restrict__ B,
-                                                       double *__restrict__ R, int nrhs)
+// R = B - A X on FP64 MFMA (v_mfma_f64_16x16x4f64), for the refinement's FP64 residuals: a
+// wave forms a 16-row x 16-column tile of A X (QT tiles side by side for nrhs <= 16 QT) over
+// one K slice; residual_sum_kernel adds the slices in order.  Within a slice the K order is
+// permuted so that each lane streams 4 consecutive elements of its row of A: at MFMA step j
+// of a 16-wide K step, lane (r, c) supplies A[r][k + 4c + j] and X[k + 4c + j][q] (any
+// fixed order of the sum is a valid residual).
+template <int QT>
+__global__ void __launch_bounds__(256) residual_mfma_kernel(const double *__restrict__ A, int lda, int n,
+                                                            const double *__restrict__ X, int nrhs, int kslice,
+                                                            double *__restrict__ part)
 {
     main_stream_priority();
-    const int row = blockIdx.x * 4 + (threadIdx.x >> 6);
-    const int lane = threadIdx.x & 63;
-    if (row >= n)
+    const int lane = threadIdx.x & 63, lr = lane & 15, lk = lane >> 4;
+    const int r0 = (blockIdx.x * 4 + (threadIdx.x >> 6)) * 16;
+    if (r0 >= n)
         return;
-    double s[kMaxRhs];
+    const int s = blockIdx.y, kb = s * kslice, ke = min(n, kb + kslice);
+    const int row = r0 + lr;
+    const double *a = A + (size_t)(row < n ? row : n - 1) * lda;
+    double4_t acc[QT];
 #pragma unroll
-    for (int q = 0; q < kMaxRhs; ++q)
-        s[q] = 0.0;
-    const TA *a = A + (size_t)row * lda;
-    for (int c = lane; c < n; c += 64)
+    for (int t = 0; t < QT; ++t)
+        acc[t] = double4_t{0, 0, 0, 0};
+    for (int k = kb; k < ke; k += 16)
     {
-        const double av = (double)a[c];
-        const double *x = X + (size_t)c * nrhs;
+        double av[4];
 #pragma unroll
-        for (int q = 0; q < kMaxRhs; ++q)
-            if (q < nrhs)
-                s[q] = fma(av, x[q], s[q]);
-    }
-#pragma unroll
-    for (int q = 0; q < kMaxRhs; ++q)
-    {
-        if (q < nrhs)
+        for (int j = 0; j < 4; ++j)
         {
-            double v = s[q];
-            for (int off = 32; off > 0; off >>= 1)
-                v += __shfl_xor(v, off);
-            if (lane == 0)
-                R[(size_t)row * nrhs + q] = B[(size_t)row * nrhs + q] - v;
+            const int kk = k + 4 * lk + j;
+            av[j] = (row < n && kk < ke) ? a[kk] : 0.0;
+        }
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+        {
+            const int kk = k + 4 * lk + j;
+#pragma unroll
+            for (int t = 0; t < QT; ++t)
+            {
+                const int q = 16 * t + lr;
+                const double bv = (kk < ke && q < nrhs) ? X[(size_t)kk * nrhs + q] : 0.0;
+                acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[j], bv, acc[t], 0, 0, 0);
+            }
         }
     }
+    // element v of lane (c, r) is (A X)[r0 + 4v + c][16t + r]
+#pragma unroll
+    for (int t = 0; t < QT; ++t)
+#pragma unroll
+        for (int v = 0; v < 4; ++v)
+        {
+            const int rr = r0 + 4 * v + lk, q = 16 * t + lr;
+            if (rr < n && q < nrhs)
+                part[((size_t)s * n + rr) * nrhs + q] = acc[t][v];
+        }
+}
+
+__global__ void __launch_bounds__(256) residual_sum_kernel(const double *__restrict__ part, int slices, size_t count,
+                                                           const double *__restrict__ B, double *__restrict__ R)
+{
+    main_stream_priority();
+    const size_t i = blockIdx.x * (size_t)256 + threadIdx.x;
+    if (i >= count)
+        return;
+    double ax = part[i];
+    for (int s = 1; s < slices; ++s)
+        ax = ax + part[(size_t)s * count + i];
+    R[i] = B[i] - ax;
 }
 
 __global__ void f64_to_f32_kernel(const double *__restrict__ a, float *__restrict__ b, size_t count)
@@ -1483,8 +1514,24 @@ int pvl_residual(int n, const double *A_dev, const double *X_dev, const double *
         set_error("residual: nrhs=%d out of range", nrhs);
         return PVL_ERR_ARG;
     }
+    // K slices: enough waves to fill the device, at least 256 columns each
+    const int row_tiles = (n + 15) / 16;
+    int slices = (8192 + row_tiles - 1) / row_tiles;
+    slices = std::max(1, std::min(slices, (n + 255) / 256));
+    const int kslice = (((n + slices - 1) / slices) + 15) & ~15;
+    slices = (n + kslice - 1) / kslice;
+    const size_t count = (size_t)n * nrhs;
+    double *part = (double *)scratch(5, sizeof(double) * count * slices);
+    if (!part)
+        return PVL_ERR_NOMEM;
     ProfScope prof("residual_f64");
-    residual_kernel<double><<<(n + 3) / 4, 256, 0, stream()>>>(A_dev, n, n, X_dev, B_dev, R_dev, nrhs);
+    const dim3 grid((row_tiles + 3) / 4, slices);
+    if (nrhs <= 16)
+        residual_mfma_kernel<1><<<grid, 256, 0, stream()>>>(A_dev, n, n, X_dev, nrhs, kslice, part);
+    else
+        residual_mfma_kernel<2><<<grid, 256, 0, stream()>>>(A_dev, n, n, X_dev, nrhs, kslice, part);
+    PVL_HIP(hipGetLastError());
+    residual_sum_kernel<<<(unsigned)((count + 255) / 256), 256, 0, stream()>>>(part, slices, count, B_dev, R_dev);
     PVL_HIP(hipGetLastError());
     return PVL_OK;
 }

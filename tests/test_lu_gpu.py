@@ -270,3 +270,16 @@ def test_unpivoted_fp32_refinement_large():
     assert np.array_equal(lu64.pivots(), np.arange(n))
     x64 = lu64.solve(b)
     assert _rel(x.numpy(), x64) < 1e-11
+
+
+@pytest.mark.parametrize("n,nrhs", [(1, 1), (37, 3), (1000, 1), (2100, 20), (4096, 32)])
+def test_residual_matches_numpy(n, nrhs):
+    """FP64 residual R = B - A X (MFMA, K slices summed in order) against numpy."""
+    rng = np.random.default_rng(300 + n)
+    a = rng.standard_normal((n, n))
+    x = rng.standard_normal((n, nrhs))
+    b = rng.standard_normal((n, nrhs))
+    r = LA.residual(DeviceArray.from_numpy(a), DeviceArray.from_numpy(x), DeviceArray.from_numpy(b)).numpy()
+    want = b - a @ x
+    # each entry is a sum of n products of size ~1: allow a few ulps of that scale
+    assert np.max(np.abs(r.reshape(want.shape) - want)) <= 1e-13 * np.sqrt(n) * 10
