@@ -849,6 +849,140 @@ __global__ void __launch_bounds__(256) trsv_step_kernel(const T *__restrict__ A,
     }
 }
 
+// Inverses of the 64 x 64 diagonal blocks of the FP32 factors (identity padding past n), for
+// the multi-launch solves inside FP64 refinement: out[(2 ib + 0)] = L_bb^-1 (unit lower),
+// out[(2 ib + 1)] = U_bb^-1, row-major 64 x 64 each.  Thread j forms column j by substitution.
+__global__ void __launch_bounds__(256) diag_inverse_kernel(const float *__restrict__ A, int lda, int n,
+                                                           float *__restrict__ out)
+{
+    main_stream_priority();
+    __shared__ float M[NB][NB + 1];
+    __shared__ float X[NB][NB + 1];
+    const int ib = blockIdx.x, upper = blockIdx.y, tid = threadIdx.x;
+    const int r0 = ib * NB, nr = min(NB, n - r0);
+    for (int idx = tid; idx < NB * NB; idx += 256)
+    {
+        const int r = idx / NB, c = idx % NB;
+        M[r][c] = (r < nr && c < nr) ? A[(size_t)(r0 + r) * lda + r0 + c] : (r == c ? 1.0f : 0.0f);
+        X[r][c] = 0.0f;
+    }
+    __syncthreads();
+    if (tid < NB)
+    {
+        const int j = tid;
+        if (!upper)
+        {
+            X[j][j] = 1.0f;
+            for (int i = j + 1; i < NB; ++i)
+            {
+                float acc = 0.0f;
+                for (int k = j; k < i; ++k)
+                    acc = fmaf(M[i][k], X[k][j], acc);
+                X[i][j] = -acc;
+            }
+        }
+        else
+        {
+            X[j][j] = 1.0f / M[j][j];
+            for (int i = j - 1; i >= 0; --i)
+            {
+                float acc = 0.0f;
+                for (int k = i + 1; k <= j; ++k)
+                    acc = fmaf(M[i][k], X[k][j], acc);
+                X[i][j] = -acc / M[i][i];
+            }
+        }
+    }
+    __syncthreads();
+    float *o = out + (size_t)(2 * ib + upper) * NB * NB;
+    for (int idx = tid; idx < NB * NB; idx += 256)
+        o[idx] = X[idx / NB][idx % NB];
+}
+
+// trsv_step_kernel for the FP32 factors with inverted diagonal blocks: the block's solve
+// is the product D^-1 acc (every entry in parallel) instead of a 64-step substitution.
+template <bool LOWER>
+__global__ void __launch_bounds__(256) trsv_step_inv_kernel(const float *__restrict__ A, int lda, int n,
+                                                            float *__restrict__ B, int nrhs, int step,
+                                                            const float *__restrict__ dinv)
+{
+    main_stream_priority<3>();
+    __shared__ float Ls[NB][NB + 1];
+    __shared__ float xs[NB][kMaxRhs + 1];
+    __shared__ float acc[NB][kMaxRhs + 1];
+    const int nblk = (n + NB - 1) / NB;
+    const int t = step + 1 + blockIdx.x; // target block, solve order
+    const int ib = LOWER ? t : nblk - 1 - t;
+    const int r0 = ib * NB, nr = min(NB, n - r0);
+    const int tid = threadIdx.x;
+    // Workgroup 0 solves its block next: fetch the block inverse now, while the dot runs.
+    constexpr int kPer = NB * NB / 256;
+    float dreg[kPer];
+    const float *D = dinv + (size_t)(2 * ib + (LOWER ? 0 : 1)) * NB * NB;
+    if (blockIdx.x == 0)
+    {
+#pragma unroll
+        for (int k = 0; k < kPer; ++k)
+            dreg[k] = D[tid + 256 * k];
+    }
+    for (int idx = tid; idx < NB * nrhs; idx += 256)
+    {
+        const int r = idx / nrhs, q = idx - r * nrhs;
+        acc[r][q] = (r < nr) ? B[(size_t)(r0 + r) * nrhs + q] : 0.0f;
+    }
+    if (step >= 0)
+    {
+        const int sb = LOWER ? step : nblk - 1 - step;
+        const int c0 = sb * NB, nc = min(NB, n - c0);
+        for (int idx = tid; idx < NB * NB; idx += 256)
+        {
+            const int r = idx / NB, c = idx % NB;
+            Ls[r][c] = (r < nr && c < nc) ? A[(size_t)(r0 + r) * lda + c0 + c] : 0.0f;
+        }
+        for (int idx = tid; idx < NB * nrhs; idx += 256)
+        {
+            const int r = idx / nrhs, q = idx - r * nrhs;
+            xs[r][q] = (r < nc) ? B[(size_t)(c0 + r) * nrhs + q] : 0.0f;
+        }
+        __syncthreads();
+        for (int idx = tid; idx < NB * nrhs; idx += 256)
+        {
+            const int r = idx / nrhs, q = idx - r * nrhs;
+            float s2 = 0.0f;
+#pragma unroll 8
+            for (int c = 0; c < NB; ++c)
+                s2 = fmaf(Ls[r][c], xs[c][q], s2);
+            acc[r][q] = acc[r][q] - s2;
+        }
+        __syncthreads();
+    }
+    if (blockIdx.x != 0)
+    {
+        for (int idx = tid; idx < nr * nrhs; idx += 256)
+        {
+            const int r = idx / nrhs, q = idx - r * nrhs;
+            B[(size_t)(r0 + r) * nrhs + q] = acc[r][q];
+        }
+        return;
+    }
+#pragma unroll
+    for (int k = 0; k < kPer; ++k)
+    {
+        const int idx = tid + 256 * k;
+        Ls[idx / NB][idx % NB] = dreg[k];
+    }
+    __syncthreads();
+    for (int idx = tid; idx < nr * nrhs; idx += 256)
+    {
+        const int r = idx / nrhs, q = idx - r * nrhs;
+        float y = 0.0f;
+#pragma unroll 8
+        for (int c = 0; c < NB; ++c)
+            y = fmaf(Ls[r][c], acc[c][q], y);
+        B[(size_t)(r0 + r) * nrhs + q] = y;
+    }
+}
+
 // Whole triangular solve in ONE workgroup (512 threads, at most 96 VGPRs: it fits on a CU
 // beside the side stream's point-velocity waves) for small systems: x (n x nrhs) lives in LDS; blocks of 64
 // rows are taken in solve order, each first reduced by the already solved part (dot products
@@ -1281,6 +1415,8 @@ struct pvl_lu
     int *perm = nullptr;   // device, n: row i of P B is row perm[i] of B
     void *work = nullptr;  // n * kMaxRhs (T) staging for permuted right-hand sides
     int pivoting = PVL_PIVOT_PARTIAL;
+    float *dinv = nullptr; // FP32: inverses of the 64 x 64 diagonal blocks of L and U
+    bool dinv_valid = false;
 };
 
 namespace
@@ -1289,6 +1425,20 @@ namespace
 template <typename T> int finish_factor(pvl_lu *h, bool interchanged)
 {
     const int n = h->n;
+    if constexpr (sizeof(T) == 4)
+    {
+        // Only solves whose right-hand sides exceed one workgroup's LDS use the inverses.
+        if ((size_t)n * kMaxRhs * sizeof(float) > (size_t)kTrsvWgBytes)
+        {
+            const int nblk = (n + NB - 1) / NB;
+            if (!h->dinv)
+                PVL_HIP(hipMalloc(&h->dinv, sizeof(float) * 2 * (size_t)nblk * NB * NB));
+            ProfScope prof("lu_diag_inverse_f32");
+            diag_inverse_kernel<<<dim3(nblk, 2), 256, 0, stream()>>>((const float *)h->lu, n, n, h->dinv);
+            PVL_HIP(hipGetLastError());
+            h->dinv_valid = true;
+        }
+    }
     if (!interchanged)
     {
         identity_kernel<<<(n + 255) / 256, 256, 0, stream()>>>(h->perm, n);
@@ -1335,6 +1485,24 @@ template <typename T> int solve_impl(pvl_lu *h, T *B, int nrhs)
             const T *Ap = (const T *)h->lu;
             trsv_wg_kernel<T, true><<<1, kTrsvWgThreads, xbytes, stream()>>>(Ap, n, n, work, nq);
             trsv_wg_kernel<T, false><<<1, kTrsvWgThreads, xbytes, stream()>>>(Ap, n, n, work, nq);
+            PVL_HIP(hipGetLastError());
+        }
+        else if constexpr (sizeof(T) == 4)
+        {
+            if (!h->dinv_valid)
+            {
+                set_error("solve: the FP32 factorisation has no diagonal-block inverses");
+                return PVL_ERR_ARG;
+            }
+            ProfScope prof("lu_trsv_f32");
+            const float *Ap = (const float *)h->lu;
+            for (int st = -1; st < nblk - 1; ++st)
+                trsv_step_inv_kernel<true><<<st < 0 ? 1 : nblk - 1 - st, 256, 0, stream()>>>(Ap, n, n, (float *)work,
+                                                                                            nq, st, h->dinv);
+            PVL_HIP(hipGetLastError());
+            for (int st = -1; st < nblk - 1; ++st)
+                trsv_step_inv_kernel<false><<<st < 0 ? 1 : nblk - 1 - st, 256, 0, stream()>>>(Ap, n, n, (float *)work,
+                                                                                             nq, st, h->dinv);
             PVL_HIP(hipGetLastError());
         }
         else
@@ -1409,6 +1577,7 @@ int pvl_lu_destroy(pvl_lu *h)
     (void)hipFree(h->piv);
     (void)hipFree(h->perm);
     (void)hipFree(h->work);
+    (void)hipFree(h->dinv);
     delete h;
     return PVL_OK;
 }
@@ -1442,6 +1611,7 @@ int pvl_lu_factor(pvl_lu *h, const double *A_dev, int *info_out)
     // result is partial pivoting's factorisation.
     const bool try_free = h->pivoting != PVL_PIVOT_PARTIAL;
     bool pivoted = false;
+    h->dinv_valid = false;
     for (int attempt = 0; attempt < 2; ++attempt)
     {
         const bool pivot = attempt == 1 || !try_free;
