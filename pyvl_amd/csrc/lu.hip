@@ -740,117 +740,8 @@ __global__ void __launch_bounds__(256) permute_rows_kernel(const T *__restrict__
     Y[idx] = B[(size_t)perm[i] * nrhs + q];
 }
 
-// Value of lane k (wave-uniform k) in every lane, through readlane (no LDS round trip).
-
-// Block triangular solve on B (n x nrhs, row-major), in place, one launch per 64-row block
-// in solve order.  LOWER: unit lower L, forward order; else upper U, backward order.
-// Launch `step` (>= 0) subtracts the contribution of the block solved by the previous
-// launch (solve-order index `step`) from every later block, and workgroup 0 then solves the
-// next block (solve-order index step + 1) with its diagonal block by substitution, as
-// LAPACK's dtrsv orders it (column sweep; lane = row).  step = -1 only solves block 0.  No workgroup waits for another: the launch
-// boundary orders the blocks.
-template <typename T, bool LOWER>
-__global__ void __launch_bounds__(256) trsv_step_kernel(const T *__restrict__ A, int lda, int n, T *__restrict__ B,
-                                                        int nrhs, int step)
-{
-    main_stream_priority<3>();
-    __shared__ T Ls[NB][NB + 1];
-    __shared__ T xs[NB][kMaxRhs + 1];
-    __shared__ T acc[NB][kMaxRhs + 1];
-    const int nblk = (n + NB - 1) / NB;
-    const int t = step + 1 + blockIdx.x; // target block, solve order
-    const int ib = LOWER ? t : nblk - 1 - t;
-    const int r0 = ib * NB, nr = min(NB, n - r0);
-    const int tid = threadIdx.x;
-    for (int idx = tid; idx < NB * nrhs; idx += 256)
-    {
-        const int r = idx / nrhs, q = idx - r * nrhs;
-        acc[r][q] = (r < nr) ? B[(size_t)(r0 + r) * nrhs + q] : T(0);
-    }
-    if (step >= 0)
-    {
-        const int sb = LOWER ? step : nblk - 1 - step;
-        const int c0 = sb * NB, nc = min(NB, n - c0);
-        for (int idx = tid; idx < NB * NB; idx += 256)
-        {
-            const int r = idx / NB, c = idx % NB;
-            Ls[r][c] = (r < nr && c < nc) ? A[(size_t)(r0 + r) * lda + c0 + c] : T(0);
-        }
-        for (int idx = tid; idx < NB * nrhs; idx += 256)
-        {
-            const int r = idx / nrhs, q = idx - r * nrhs;
-            xs[r][q] = (r < nc) ? B[(size_t)(c0 + r) * nrhs + q] : T(0);
-        }
-        __syncthreads();
-        // the block's contribution as a dot product, subtracted once (as a blocked dtrsv
-        // with a GEMV update does; folding each term into the running value loses accuracy
-        // over hundreds of blocks)
-        for (int idx = tid; idx < NB * nrhs; idx += 256)
-        {
-            const int r = idx / nrhs, q = idx - r * nrhs;
-            T s2 = T(0);
-#pragma unroll 8
-            for (int c = 0; c < NB; ++c)
-                s2 = fma(Ls[r][c], xs[c][q], s2);
-            acc[r][q] = acc[r][q] - s2;
-        }
-        __syncthreads();
-    }
-    if (blockIdx.x != 0)
-    {
-        for (int idx = tid; idx < nr * nrhs; idx += 256)
-        {
-            const int r = idx / nrhs, q = idx - r * nrhs;
-            B[(size_t)(r0 + r) * nrhs + q] = acc[r][q];
-        }
-        return;
-    }
-    // this block is complete: x = D^-1 acc (identity padding past n)
-    for (int idx = tid; idx < NB * NB; idx += 256)
-    {
-        const int r = idx / NB, c = idx % NB;
-        Ls[r][c] = (r < nr && c < nr) ? A[(size_t)(r0 + r) * lda + r0 + c] : (r == c ? T(1) : T(0));
-    }
-    __syncthreads();
-    // wave wv holds the right-hand sides wv, wv + 4, ... (at most kMaxRhs / 4) in registers
-    // and advances them together; the current row's value is broadcast with readlane.
-    const int lane = tid & 63, wv = tid >> 6;
-    constexpr int QW = kMaxRhs / 4;
-    T x[QW];
-#pragma unroll
-    for (int j = 0; j < QW; ++j)
-    {
-        const int q = wv + 4 * j;
-        x[j] = q < nrhs ? acc[lane][q] : T(0);
-    }
-    const T lrow_diag = Ls[lane][lane];
-    for (int kk = 0; kk < NB; ++kk)
-    {
-        const int k = LOWER ? kk : NB - 1 - kk;
-        const T lk = Ls[lane][k];
-#pragma unroll
-        for (int j = 0; j < QW; ++j)
-        {
-            if (wv + 4 * j >= nrhs) // uniform
-                break;
-            if (!LOWER && lane == k)
-                x[j] = x[j] / lrow_diag;
-            const T xk = bcast_lane(x[j], k);
-            if (LOWER ? lane > k : lane < k)
-                x[j] = fma(-lk, xk, x[j]);
-        }
-    }
-#pragma unroll
-    for (int j = 0; j < QW; ++j)
-    {
-        const int q = wv + 4 * j;
-        if (q < nrhs && lane < nr)
-            B[(size_t)(r0 + lane) * nrhs + q] = x[j];
-    }
-}
-
 // Inverses of the 64 x 64 diagonal blocks of the FP32 factors (identity padding past n), for
-// the multi-launch solves inside FP64 refinement: out[(2 ib + 0)] = L_bb^-1 (unit lower),
+// the single-launch solves (inside FP64 refinement): out[(2 ib + 0)] = L_bb^-1 (unit lower),
 // out[(2 ib + 1)] = U_bb^-1, row-major 64 x 64 each.  Thread j forms column j by substitution.
 __global__ void __launch_bounds__(256) diag_inverse_kernel(const float *__restrict__ A, int lda, int n,
                                                            float *__restrict__ out)
@@ -899,88 +790,222 @@ __global__ void __launch_bounds__(256) diag_inverse_kernel(const float *__restri
         o[idx] = X[idx / NB][idx % NB];
 }
 
-// trsv_step_kernel for the FP32 factors with inverted diagonal blocks: the block's solve
-// is the product D^-1 acc (every entry in parallel) instead of a 64-step substitution.
-template <bool LOWER>
-__global__ void __launch_bounds__(256) trsv_step_inv_kernel(const float *__restrict__ A, int lda, int n,
-                                                            float *__restrict__ B, int nrhs, int step,
-                                                            const float *__restrict__ dinv)
+// Block triangular solve on B (n x nrhs, row-major), in place, in ONE launch of one
+// workgroup per 64-row block.  LOWER: unit lower L, forward order; else upper U, backward
+// order.  Workgroups take solve-order indices from a ticket counter, so every smaller index
+// belongs to a workgroup already running; workgroup t subtracts the contributions of blocks
+// 0 .. t-1 in that order as their flags (sync[kFlowHeader + k] == epoch) appear, each block
+// as a dot product subtracted once (as a blocked dtrsv with a GEMV update does; folding each
+// term into the running value loses accuracy over hundreds of blocks), then solves its own
+// block and raises its flag.  The diagonal solve is, for FP32, the product with the block's
+// precomputed inverse; for FP64, substitution in LAPACK dtrsv's order (column sweep, lane =
+// row).  Waits are bounded: a timed-out wait sets *err and the workgroup writes NaN.
+constexpr int kFlowHeader = 64; // sync[0], sync[1]: tickets of the lower and upper solves
+
+__device__ __forceinline__ bool flow_wait(const unsigned *flag, unsigned epoch, int *err)
+{
+    // relaxed polling (device-coherent loads, no cache invalidation per poll), one acquire
+    // fence once the flag is seen
+    unsigned spins = 0;
+    bool ok = true;
+    while (__hip_atomic_load(flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != epoch)
+    {
+        __builtin_amdgcn_s_sleep(1);
+        if (++spins > kSpinLimit ||
+            ((spins & 1023u) == 0 && __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)))
+        {
+            __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            ok = false;
+            break;
+        }
+    }
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    return ok;
+}
+
+__global__ void flow_reset_kernel(unsigned *__restrict__ sync, int *__restrict__ err)
+{
+    main_stream_priority();
+    if (threadIdx.x < 2)
+        sync[threadIdx.x] = 0u;
+    if (threadIdx.x == 2)
+        *err = 0;
+}
+
+constexpr int kXs = kMaxRhs + 4; // row stride of the right-hand-side tiles (16-byte rows)
+
+// out(r, q..q+3) = sum_c M[r][c] X[c][q..q+3] for the 64 rows and nq4 groups of 4 columns,
+// each sum in column order c = 0, 1, ... (one thread per row and column group)
+template <typename T, typename F>
+__device__ __forceinline__ void block_product(const T (*M)[NB + 1], const T (*X)[kXs], int nq4, F &&out)
+{
+    for (int it = threadIdx.x; it < NB * nq4; it += 256)
+    {
+        const int r = it / nq4, q = 4 * (it - r * nq4);
+        T s0 = T(0), s1 = T(0), s2 = T(0), s3 = T(0);
+#pragma unroll 16
+        for (int c = 0; c < NB; ++c)
+        {
+            const T m = M[r][c];
+            s0 = fma(m, X[c][q], s0);
+            s1 = fma(m, X[c][q + 1], s1);
+            s2 = fma(m, X[c][q + 2], s2);
+            s3 = fma(m, X[c][q + 3], s3);
+        }
+        out(r, q, s0, s1, s2, s3);
+    }
+}
+
+template <typename T, bool LOWER>
+__global__ void __launch_bounds__(256) trsv_flow_kernel(const T *__restrict__ A, int lda, int n, T *__restrict__ B,
+                                                        int nrhs, const float *__restrict__ dinv,
+                                                        unsigned *__restrict__ sync, unsigned epoch,
+                                                        int *__restrict__ err)
 {
     main_stream_priority<3>();
-    __shared__ float Ls[NB][NB + 1];
-    __shared__ float xs[NB][kMaxRhs + 1];
-    __shared__ float acc[NB][kMaxRhs + 1];
+    __shared__ T Ls[NB][NB + 1];
+    __shared__ __align__(16) T xs[NB][kXs];
+    __shared__ __align__(16) T acc[NB][kXs];
+    __shared__ int s_t, s_ok;
+    const int tid = threadIdx.x;
+    if (tid == 0)
+    {
+        s_t = (int)__hip_atomic_fetch_add(&sync[LOWER ? 0 : 1], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        s_ok = 1;
+    }
+    __syncthreads();
+    unsigned *flag = sync + kFlowHeader;
     const int nblk = (n + NB - 1) / NB;
-    const int t = step + 1 + blockIdx.x; // target block, solve order
+    const int t = s_t; // solve order
     const int ib = LOWER ? t : nblk - 1 - t;
     const int r0 = ib * NB, nr = min(NB, n - r0);
-    const int tid = threadIdx.x;
-    // Workgroup 0 solves its block next: fetch the block inverse now, while the dot runs.
+    const int nq4 = (nrhs + 3) / 4, nqp = 4 * nq4;
     constexpr int kPer = NB * NB / 256;
-    float dreg[kPer];
-    const float *D = dinv + (size_t)(2 * ib + (LOWER ? 0 : 1)) * NB * NB;
-    if (blockIdx.x == 0)
-    {
-#pragma unroll
-        for (int k = 0; k < kPer; ++k)
-            dreg[k] = D[tid + 256 * k];
-    }
-    for (int idx = tid; idx < NB * nrhs; idx += 256)
-    {
-        const int r = idx / nrhs, q = idx - r * nrhs;
-        acc[r][q] = (r < nr) ? B[(size_t)(r0 + r) * nrhs + q] : 0.0f;
-    }
-    if (step >= 0)
-    {
-        const int sb = LOWER ? step : nblk - 1 - step;
-        const int c0 = sb * NB, nc = min(NB, n - c0);
-        for (int idx = tid; idx < NB * NB; idx += 256)
-        {
-            const int r = idx / NB, c = idx % NB;
-            Ls[r][c] = (r < nr && c < nc) ? A[(size_t)(r0 + r) * lda + c0 + c] : 0.0f;
-        }
-        for (int idx = tid; idx < NB * nrhs; idx += 256)
-        {
-            const int r = idx / nrhs, q = idx - r * nrhs;
-            xs[r][q] = (r < nc) ? B[(size_t)(c0 + r) * nrhs + q] : 0.0f;
-        }
-        __syncthreads();
-        for (int idx = tid; idx < NB * nrhs; idx += 256)
-        {
-            const int r = idx / nrhs, q = idx - r * nrhs;
-            float s2 = 0.0f;
-#pragma unroll 8
-            for (int c = 0; c < NB; ++c)
-                s2 = fmaf(Ls[r][c], xs[c][q], s2);
-            acc[r][q] = acc[r][q] - s2;
-        }
-        __syncthreads();
-    }
-    if (blockIdx.x != 0)
-    {
-        for (int idx = tid; idx < nr * nrhs; idx += 256)
-        {
-            const int r = idx / nrhs, q = idx - r * nrhs;
-            B[(size_t)(r0 + r) * nrhs + q] = acc[r][q];
-        }
-        return;
-    }
+    // the diagonal block's operator (inverse for FP32, the factor block for FP64) and the
+    // first off-diagonal block go to registers now; later blocks are fetched one ahead and
+    // staged in LDS before the wait for their right-hand side
+    T dreg[kPer], lreg[kPer];
 #pragma unroll
     for (int k = 0; k < kPer; ++k)
     {
-        const int idx = tid + 256 * k;
-        Ls[idx / NB][idx % NB] = dreg[k];
+        const int idx = tid + 256 * k, r = idx / NB, c = idx % NB;
+        if constexpr (sizeof(T) == 4)
+            dreg[k] = dinv[(size_t)(2 * ib + (LOWER ? 0 : 1)) * NB * NB + idx];
+        else
+            dreg[k] = (r < nr && c < nr) ? A[(size_t)(r0 + r) * lda + r0 + c] : (r == c ? T(1) : T(0));
     }
-    __syncthreads();
-    for (int idx = tid; idx < nr * nrhs; idx += 256)
+    auto fetch_l = [&](int k) {
+        const int c0 = (LOWER ? k : nblk - 1 - k) * NB, nc = min(NB, n - c0);
+#pragma unroll
+        for (int kk = 0; kk < kPer; ++kk)
+        {
+            const int idx = tid + 256 * kk, r = idx / NB, c = idx % NB;
+            lreg[kk] = (r < nr && c < nc) ? A[(size_t)(r0 + r) * lda + c0 + c] : T(0);
+        }
+    };
+    auto stage = [&](const T *reg) {
+#pragma unroll
+        for (int kk = 0; kk < kPer; ++kk)
+        {
+            const int idx = tid + 256 * kk;
+            Ls[idx / NB][idx % NB] = reg[kk];
+        }
+    };
+    for (int idx = tid; idx < NB * nqp; idx += 256)
     {
-        const int r = idx / nrhs, q = idx - r * nrhs;
-        float y = 0.0f;
-#pragma unroll 8
-        for (int c = 0; c < NB; ++c)
-            y = fmaf(Ls[r][c], acc[c][q], y);
-        B[(size_t)(r0 + r) * nrhs + q] = y;
+        const int r = idx / nqp, q = idx - r * nqp;
+        acc[r][q] = (r < nr && q < nrhs) ? B[(size_t)(r0 + r) * nrhs + q] : T(0);
     }
+    if (t > 0)
+    {
+        fetch_l(0);
+        stage(lreg);
+        if (t > 1)
+            fetch_l(1);
+    }
+    for (int k = 0; k < t; ++k)
+    {
+        const int c0 = (LOWER ? k : nblk - 1 - k) * NB, nc = min(NB, n - c0);
+        if (tid == 0 && !flow_wait(&flag[k], epoch, err))
+            s_ok = 0;
+        __syncthreads();
+        for (int idx = tid; idx < NB * nqp; idx += 256)
+        {
+            const int r = idx / nqp, q = idx - r * nqp;
+            xs[r][q] = (r < nc && q < nrhs) ? xload(&B[(size_t)(c0 + r) * nrhs + q]) : T(0);
+        }
+        __syncthreads();
+        block_product<T>(Ls, xs, nq4, [&](int r, int q, T a0, T a1, T a2, T a3) {
+            acc[r][q] -= a0;
+            acc[r][q + 1] -= a1;
+            acc[r][q + 2] -= a2;
+            acc[r][q + 3] -= a3;
+        });
+        __syncthreads();
+        if (k + 1 < t)
+        {
+            stage(lreg);
+            if (k + 2 < t)
+                fetch_l(k + 2);
+        }
+    }
+    stage(dreg);
+    __syncthreads();
+    const T poison = s_ok ? T(0) : T(NAN);
+    if constexpr (sizeof(T) == 4)
+    {
+        block_product<T>(Ls, acc, nq4, [&](int r, int q, T a0, T a1, T a2, T a3) {
+            if (r < nr)
+            {
+                const T v[4] = {a0, a1, a2, a3};
+                for (int j = 0; j < 4 && q + j < nrhs; ++j)
+                    xstore(&B[(size_t)(r0 + r) * nrhs + q + j], v[j] + poison);
+            }
+        });
+    }
+    else
+    {
+        // wave wv holds the right-hand sides wv, wv + 4, ... (at most kMaxRhs / 4) in
+        // registers and advances them together; the current row's value is broadcast with
+        // readlane
+        const int lane = tid & 63, wv = tid >> 6;
+        constexpr int QW = kMaxRhs / 4;
+        T x[QW];
+#pragma unroll
+        for (int j = 0; j < QW; ++j)
+        {
+            const int q = wv + 4 * j;
+            x[j] = q < nrhs ? acc[lane][q] : T(0);
+        }
+        const T lrow_diag = Ls[lane][lane];
+        for (int kk = 0; kk < NB; ++kk)
+        {
+            const int k = LOWER ? kk : NB - 1 - kk;
+            const T lk = Ls[lane][k];
+#pragma unroll
+            for (int j = 0; j < QW; ++j)
+            {
+                if (wv + 4 * j >= nrhs) // uniform
+                    break;
+                if (!LOWER && lane == k)
+                    x[j] = x[j] / lrow_diag;
+                const T xk = bcast_lane(x[j], k);
+                if (LOWER ? lane > k : lane < k)
+                    x[j] = fma(-lk, xk, x[j]);
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < QW; ++j)
+        {
+            const int q = wv + 4 * j;
+            if (q < nrhs && lane < nr)
+                xstore(&B[(size_t)(r0 + lane) * nrhs + q], x[j] + poison);
+        }
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    __syncthreads();
+    if (tid == 0)
+        __hip_atomic_store(&flag[t], epoch, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // Whole triangular solve in ONE workgroup (512 threads, at most 96 VGPRs: it fits on a CU
@@ -1417,6 +1442,8 @@ struct pvl_lu
     int pivoting = PVL_PIVOT_PARTIAL;
     float *dinv = nullptr; // FP32: inverses of the 64 x 64 diagonal blocks of L and U
     bool dinv_valid = false;
+    unsigned *flow = nullptr; // tickets and block flags of the single-launch solves
+    unsigned epoch = 0;       // flag value of the latest solve launch
 };
 
 namespace
@@ -1487,33 +1514,27 @@ template <typename T> int solve_impl(pvl_lu *h, T *B, int nrhs)
             trsv_wg_kernel<T, false><<<1, kTrsvWgThreads, xbytes, stream()>>>(Ap, n, n, work, nq);
             PVL_HIP(hipGetLastError());
         }
-        else if constexpr (sizeof(T) == 4)
+        else
         {
-            if (!h->dinv_valid)
+            if (sizeof(T) == 4 && !h->dinv_valid)
             {
                 set_error("solve: the FP32 factorisation has no diagonal-block inverses");
                 return PVL_ERR_ARG;
             }
-            ProfScope prof("lu_trsv_f32");
-            const float *Ap = (const float *)h->lu;
-            for (int st = -1; st < nblk - 1; ++st)
-                trsv_step_inv_kernel<true><<<st < 0 ? 1 : nblk - 1 - st, 256, 0, stream()>>>(Ap, n, n, (float *)work,
-                                                                                            nq, st, h->dinv);
-            PVL_HIP(hipGetLastError());
-            for (int st = -1; st < nblk - 1; ++st)
-                trsv_step_inv_kernel<false><<<st < 0 ? 1 : nblk - 1 - st, 256, 0, stream()>>>(Ap, n, n, (float *)work,
-                                                                                             nq, st, h->dinv);
-            PVL_HIP(hipGetLastError());
-        }
-        else
-        {
             ProfScope prof(sizeof(T) == 8 ? "lu_trsv_f64" : "lu_trsv_f32");
             const T *Ap = (const T *)h->lu;
-            for (int st = -1; st < nblk - 1; ++st)
-                trsv_step_kernel<T, true><<<st < 0 ? 1 : nblk - 1 - st, 256, 0, stream()>>>(Ap, n, n, work, nq, st);
-            PVL_HIP(hipGetLastError());
-            for (int st = -1; st < nblk - 1; ++st)
-                trsv_step_kernel<T, false><<<st < 0 ? 1 : nblk - 1 - st, 256, 0, stream()>>>(Ap, n, n, work, nq, st);
+            flow_reset_kernel<<<1, 64, 0, stream()>>>(h->flow, g_ws.err + 5);
+            for (int pass = 0; pass < 2; ++pass)
+            {
+                if (++h->epoch == 0)
+                    h->epoch = 1;
+                if (pass == 0)
+                    trsv_flow_kernel<T, true><<<nblk, 256, 0, stream()>>>(Ap, n, n, work, nq, h->dinv, h->flow,
+                                                                         h->epoch, g_ws.err + 5);
+                else
+                    trsv_flow_kernel<T, false><<<nblk, 256, 0, stream()>>>(Ap, n, n, work, nq, h->dinv, h->flow,
+                                                                          h->epoch, g_ws.err + 5);
+            }
             PVL_HIP(hipGetLastError());
         }
         if (int e = device_copy(B, work, sizeof(T) * (size_t)n * nq))
@@ -1528,7 +1549,9 @@ int alloc_handle(pvl_lu *h, int n, int prec)
     PVL_HIP(hipMalloc(&h->piv, sizeof(int) * n));
     PVL_HIP(hipMalloc(&h->perm, sizeof(int) * n));
     PVL_HIP(hipMalloc(&h->work, es * (size_t)n * kMaxRhs));
-    return PVL_OK;
+    const int nblk = (n + NB - 1) / NB;
+    PVL_HIP(hipMalloc(&h->flow, sizeof(unsigned) * (kFlowHeader + nblk)));
+    return device_fill32(h->flow, 0u, sizeof(unsigned) * (kFlowHeader + nblk));
 }
 
 } // namespace
@@ -1578,6 +1601,7 @@ int pvl_lu_destroy(pvl_lu *h)
     (void)hipFree(h->perm);
     (void)hipFree(h->work);
     (void)hipFree(h->dinv);
+    (void)hipFree(h->flow);
     delete h;
     return PVL_OK;
 }
