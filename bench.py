@@ -428,6 +428,9 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true", help="skip the host baselines")
     ap.add_argument("--headline-only", action="store_true", help="only the FP64 free-wake workload")
+    ap.add_argument("--solve-only", action="store_true",
+                    help="only the 16k-panel closed-body solve (FP64 LU, FP32 LU + FP64 refinement); prints its "
+                         "own JSON line, not the headline one")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="first copy 1 GiB device-to-device (a plain copy kernel of known size, the reference "
                          "for scaling hardware byte counters)")
@@ -439,6 +442,9 @@ def main() -> int:
     from pyvl_amd import _lib
 
     _lib.call("pvl_init", 0)
+    if args.solve_only:
+        print(json.dumps({"workload": "16k-panel closed-body solve, 16 RHS", **run_c4(not args.no_cpu)}), flush=True)
+        return 0
     if args.copy_calibration:
         from pyvl_amd import DeviceArray
 
