@@ -182,7 +182,11 @@ int pvl_lu_set_pivoting(pvl_lu *h, int pivoting);
  * PVL_ERR_SINGULAR with *info_out = first zero pivot (1-based) when U is exactly singular. */
 int pvl_lu_factor(pvl_lu *h, const double *A_dev, int *info_out);
 /* Solve in the handle's precision: B_dev (double* for FP64 handles, float* for FP32) is
- * overwritten with the solution.  1 <= nrhs <= 32. */
+ * overwritten with the solution.  1 <= nrhs <= 32.  Asynchronous on the library stream;
+ * solves of one handle are ordered by that stream (they share its flag and staging
+ * buffers).  Large systems run as one launch per triangle whose workgroups wait on each
+ * other with bounded waits: should a wait time out (about a second), the affected rows are
+ * written as NaN rather than left stale. */
 int pvl_lu_solve(pvl_lu *h, void *B_dev, int nrhs);
 int pvl_lu_pivots(const pvl_lu *h, int *piv_host);
 int pvl_lu_factors(const pvl_lu *h, void *lu_dev_out);
