@@ -618,8 +618,10 @@ __global__ void __launch_bounds__(256) trsm_kernel(T *__restrict__ A, int lda, i
 // and B[l/16][l%16]; the accumulator row of element v is Mfma<T>::out_row.  Each 64-wide K
 // segment (one panel's L21 U12) is summed from zero and subtracted from C once (as dgemm's
 // C - A B: one rounding at |C|'s magnitude per panel, not one per k on diagonally dominant
-// updates); the C tile stays in registers from the start to the end, across segments.  K is staged through LDS in chunks of 32 (33 KB for FP64: four
-// workgroups per CU keep the matrix cores fed while others wait on memory).
+// updates); the C tile stays in registers from the start to the end, across segments.  K is
+// staged through LDS in chunks of 32, software-pipelined: the next chunk's operands are
+// loaded into registers while the current chunk feeds the MFMAs, so global latency overlaps
+// the matrix work (FP64: 164 VGPRs, three workgroups per CU; FP32: four).
 // Tile order (1-D grid): consecutive workgroups go round-robin to the 8 XCDs, each with its
 // own L2, so workgroup g works for XCD g % 8, which owns a stripe of cpx column tiles and
 // sweeps it row tile by row tile: its U12 stripe stays in its L2 and each L21 tile is
@@ -628,7 +630,7 @@ constexpr int kXcds = 8;
 // C = rows [row0, row0 + m2) x columns [col0, col0 + n2) of A; the update subtracts
 // A[C rows, kk0 .. kk0 + K) times A[kk0 .. kk0 + K, C columns].
 template <typename T>
-__global__ void __launch_bounds__(256) gemm_kernel(T *__restrict__ A, int lda, int row0, int m2, int col0, int n2,
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 8 ? 3 : 4))) gemm_kernel(T *__restrict__ A, int lda, int row0, int m2, int col0, int n2,
                                                    int kk0, int K, int cpx)
 {
     main_stream_priority<2>();
@@ -666,21 +668,43 @@ __global__ void __launch_bounds__(256) gemm_kernel(T *__restrict__ A, int lda, i
 #pragma unroll
         for (int b = 0; b < 2; ++b)
             acc[a][b] = acc_t{0, 0, 0, 0};
+    // software pipeline: chunk k0 + KC is fetched into registers while chunk k0 is multiplied
+    constexpr int PA = BM * KC / 256, PB = KC * BN / 256;
+    T ra[PA], rb[PB];
+    auto fetch = [&](int c0) {
+#pragma unroll
+        for (int i = 0; i < PA; ++i)
+        {
+            const int idx = tid + 256 * i, r = idx / KC, t = idx % KC;
+            ra[i] = (i0 + r < m2 && c0 + t < jb) ? A[(size_t)(rowA + i0 + r) * lda + k + c0 + t] : T(0);
+        }
+#pragma unroll
+        for (int i = 0; i < PB; ++i)
+        {
+            const int idx = tid + 256 * i, t = idx / BN, c = idx % BN;
+            rb[i] = (c0 + t < jb && j0 + c < n2) ? A[(size_t)(k + c0 + t) * lda + colB + j0 + c] : T(0);
+        }
+    };
+    fetch(0);
     for (int k0 = 0; k0 < jb; k0 += KC)
     {
         if (k0)
             __syncthreads(); // the previous chunk's operands are consumed
-        for (int idx = tid; idx < BM * KC; idx += 256)
+#pragma unroll
+        for (int i = 0; i < PA; ++i)
         {
-            const int r = idx / KC, t = idx % KC;
-            As[r][t] = (i0 + r < m2 && k0 + t < jb) ? A[(size_t)(rowA + i0 + r) * lda + k + k0 + t] : T(0);
+            const int idx = tid + 256 * i;
+            As[idx / KC][idx % KC] = ra[i];
         }
-        for (int idx = tid; idx < KC * BN; idx += 256)
+#pragma unroll
+        for (int i = 0; i < PB; ++i)
         {
-            const int t = idx / BN, c = idx % BN;
-            Bs[t][c] = (k0 + t < jb && j0 + c < n2) ? A[(size_t)(k + k0 + t) * lda + colB + j0 + c] : T(0);
+            const int idx = tid + 256 * i;
+            Bs[idx / BN][idx % BN] = rb[i];
         }
         __syncthreads();
+        if (k0 + KC < jb)
+            fetch(k0 + KC);
         const int kend = min(KC, (jb - k0 + 3) & ~3);
         for (int kk = 0; kk < kend; kk += 4)
         {
