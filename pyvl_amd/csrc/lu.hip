@@ -22,6 +22,7 @@
 
 #include <climits>
 #include <cmath>
+#include <type_traits>
 #include <utility>
 #include <vector>
 
@@ -629,7 +630,19 @@ __global__ void __launch_bounds__(256) trsm_kernel(T *__restrict__ A, int lda, i
 constexpr int kXcds = 8;
 // C = rows [row0, row0 + m2) x columns [col0, col0 + n2) of A; the update subtracts
 // A[C rows, kk0 .. kk0 + K) times A[kk0 .. kk0 + K, C columns].
-template <typename T>
+// 16-byte vector of T (one global_load_dwordx4 / ds_write_b128)
+template <typename T> struct Vec16;
+template <> struct Vec16<double>
+{
+    typedef double type __attribute__((ext_vector_type(2)));
+};
+template <> struct Vec16<float>
+{
+    typedef float type __attribute__((ext_vector_type(4)));
+};
+template <typename T> using V16 = typename Vec16<T>::type;
+
+template <typename T, bool AL>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 8 ? 3 : 4))) gemm_kernel(T *__restrict__ A, int lda, int row0, int m2, int col0, int n2,
                                                    int kk0, int K, int cpx)
 {
@@ -637,8 +650,8 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof
     constexpr int BM = 64, BN = 64, KC = 32;
     // Row strides 36 and 80 elements: the operand reads (16 rows x 4 k, 4 k x 16 columns per
     // wave) then fall on distinct LDS banks apart from the two passes a 64-lane read needs.
-    __shared__ T As[BM][KC + 4];
-    __shared__ T Bs[KC][BN + 16];
+    __shared__ __align__(16) T As[BM][KC + 4];
+    __shared__ __align__(16) T Bs[KC][BN + 16];
     const int tid = threadIdx.x;
     const int xcd = blockIdx.x % kXcds, li = blockIdx.x / kXcds;
     const int tr = li / cpx, tc = xcd * cpx + li % cpx;
@@ -668,21 +681,48 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof
 #pragma unroll
         for (int b = 0; b < 2; ++b)
             acc[a][b] = acc_t{0, 0, 0, 0};
-    // software pipeline: chunk k0 + KC is fetched into registers while chunk k0 is multiplied
-    constexpr int PA = BM * KC / 256, PB = KC * BN / 256;
-    T ra[PA], rb[PB];
+    // software pipeline: chunk k0 + KC is fetched into registers while chunk k0 is multiplied.
+    // FP32 with an aligned lda (VL) loads 16 bytes per access (element loads at the matrix
+    // edges); FP64 loads elements (its 16-byte variant measured slower).
+    constexpr bool VL = AL && sizeof(T) == 4;
+    constexpr int VEC = VL ? 16 / (int)sizeof(T) : 1;
+    using R = std::conditional_t<VL, V16<T>, T>;
+    constexpr int PA = BM * KC / 256 / VEC, PB = KC * BN / 256 / VEC;
+    R ra[PA], rb[PB];
     auto fetch = [&](int c0) {
 #pragma unroll
         for (int i = 0; i < PA; ++i)
         {
-            const int idx = tid + 256 * i, r = idx / KC, t = idx % KC;
-            ra[i] = (i0 + r < m2 && c0 + t < jb) ? A[(size_t)(rowA + i0 + r) * lda + k + c0 + t] : T(0);
+            const int idx = tid + 256 * i, r = idx / (KC / VEC), t = (idx % (KC / VEC)) * VEC;
+            const T *src = A + (size_t)(rowA + i0 + r) * lda + k + c0 + t;
+            if constexpr (VL)
+            {
+                if (i0 + r < m2 && c0 + t + VEC <= jb)
+                    ra[i] = *reinterpret_cast<const R *>(src);
+                else
+#pragma unroll
+                    for (int e = 0; e < VEC; ++e)
+                        ra[i][e] = (i0 + r < m2 && c0 + t + e < jb) ? src[e] : T(0);
+            }
+            else
+                ra[i] = (i0 + r < m2 && c0 + t < jb) ? *src : T(0);
         }
 #pragma unroll
         for (int i = 0; i < PB; ++i)
         {
-            const int idx = tid + 256 * i, t = idx / BN, c = idx % BN;
-            rb[i] = (c0 + t < jb && j0 + c < n2) ? A[(size_t)(k + c0 + t) * lda + colB + j0 + c] : T(0);
+            const int idx = tid + 256 * i, t = idx / (BN / VEC), c = (idx % (BN / VEC)) * VEC;
+            const T *src = A + (size_t)(k + c0 + t) * lda + colB + j0 + c;
+            if constexpr (VL)
+            {
+                if (c0 + t < jb && j0 + c + VEC <= n2)
+                    rb[i] = *reinterpret_cast<const R *>(src);
+                else
+#pragma unroll
+                    for (int e = 0; e < VEC; ++e)
+                        rb[i][e] = (c0 + t < jb && j0 + c + e < n2) ? src[e] : T(0);
+            }
+            else
+                rb[i] = (c0 + t < jb && j0 + c < n2) ? *src : T(0);
         }
     };
     fetch(0);
@@ -694,13 +734,13 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof
         for (int i = 0; i < PA; ++i)
         {
             const int idx = tid + 256 * i;
-            As[idx / KC][idx % KC] = ra[i];
+            *reinterpret_cast<R *>(&As[idx / (KC / VEC)][(idx % (KC / VEC)) * VEC]) = ra[i];
         }
 #pragma unroll
         for (int i = 0; i < PB; ++i)
         {
             const int idx = tid + 256 * i;
-            Bs[idx / BN][idx % BN] = rb[i];
+            *reinterpret_cast<R *>(&Bs[idx / (BN / VEC)][(idx % (BN / VEC)) * VEC]) = rb[i];
         }
         __syncthreads();
         if (k0 + KC < jb)
@@ -1328,7 +1368,14 @@ template <typename T> int factor_impl(int n, T *A, int *piv, int *info_out, bool
         const int nt = (n2 + 63) / 64, cpx = (nt + kXcds - 1) / kXcds;
         const int mt = (m2 + 63) / 64;
         ProfScope prof(sizeof(T) == 8 ? "lu_gemm_f64" : "lu_gemm_f32");
-        gemm_kernel<T><<<kXcds * mt * cpx, 256, 0, stream()>>>(A, n, row0, m2, col0, n2, kk0, K, cpx);
+        bool vec = false;
+        if constexpr (sizeof(T) == 4)
+            vec = n % 4 == 0;
+        if (vec)
+            gemm_kernel<float, true><<<kXcds * mt * cpx, 256, 0, stream()>>>((float *)A, n, row0, m2, col0, n2, kk0,
+                                                                             K, cpx);
+        else
+            gemm_kernel<T, false><<<kXcds * mt * cpx, 256, 0, stream()>>>(A, n, row0, m2, col0, n2, kk0, K, cpx);
         PVL_HIP(hipGetLastError());
         return PVL_OK;
     };
