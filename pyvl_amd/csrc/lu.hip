@@ -804,31 +804,32 @@ __global__ void __launch_bounds__(256) permute_rows_kernel(const T *__restrict__
     Y[idx] = B[(size_t)perm[i] * nrhs + q];
 }
 
-// Inverses of the 64 x 64 diagonal blocks of the FP32 factors (identity padding past n), for
+// Inverses of the BS x BS diagonal blocks of the FP32 factors (identity padding past n), for
 // the single-launch solves (inside FP64 refinement): out[(2 ib + 0)] = L_bb^-1 (unit lower),
-// out[(2 ib + 1)] = U_bb^-1, row-major 64 x 64 each.  Thread j forms column j by substitution.
+// out[(2 ib + 1)] = U_bb^-1, row-major BS x BS each.  Thread j forms column j by substitution.
+template <int BS>
 __global__ void __launch_bounds__(256) diag_inverse_kernel(const float *__restrict__ A, int lda, int n,
                                                            float *__restrict__ out)
 {
     main_stream_priority();
-    __shared__ float M[NB][NB + 1];
-    __shared__ float X[NB][NB + 1];
+    __shared__ float M[BS][BS + 1];
+    __shared__ float X[BS][BS + 1];
     const int ib = blockIdx.x, upper = blockIdx.y, tid = threadIdx.x;
-    const int r0 = ib * NB, nr = min(NB, n - r0);
-    for (int idx = tid; idx < NB * NB; idx += 256)
+    const int r0 = ib * BS, nr = min(BS, n - r0);
+    for (int idx = tid; idx < BS * BS; idx += 256)
     {
-        const int r = idx / NB, c = idx % NB;
+        const int r = idx / BS, c = idx % BS;
         M[r][c] = (r < nr && c < nr) ? A[(size_t)(r0 + r) * lda + r0 + c] : (r == c ? 1.0f : 0.0f);
         X[r][c] = 0.0f;
     }
     __syncthreads();
-    if (tid < NB)
+    if (tid < BS)
     {
         const int j = tid;
         if (!upper)
         {
             X[j][j] = 1.0f;
-            for (int i = j + 1; i < NB; ++i)
+            for (int i = j + 1; i < BS; ++i)
             {
                 float acc = 0.0f;
                 for (int k = j; k < i; ++k)
@@ -849,9 +850,9 @@ __global__ void __launch_bounds__(256) diag_inverse_kernel(const float *__restri
         }
     }
     __syncthreads();
-    float *o = out + (size_t)(2 * ib + upper) * NB * NB;
-    for (int idx = tid; idx < NB * NB; idx += 256)
-        o[idx] = X[idx / NB][idx % NB];
+    float *o = out + (size_t)(2 * ib + upper) * BS * BS;
+    for (int idx = tid; idx < BS * BS; idx += 256)
+        o[idx] = X[idx / BS][idx % BS];
 }
 
 // Block triangular solve on B (n x nrhs, row-major), in place, in ONE launch of one
@@ -897,18 +898,19 @@ __global__ void flow_reset_kernel(unsigned *__restrict__ sync, int *__restrict__
 }
 
 constexpr int kXs = kMaxRhs + 4; // row stride of the right-hand-side tiles (16-byte rows)
+constexpr int kFlowBs32 = 128;   // block rows of the FP32 solves (and of their diagonal inverses)
 
-// out(r, q..q+3) = sum_c M[r][c] X[c][q..q+3] for the 64 rows and nq4 groups of 4 columns,
+// out(r, q..q+3) = sum_c M[r][c] X[c][q..q+3] for the BS rows and nq4 groups of 4 columns,
 // each sum in column order c = 0, 1, ... (one thread per row and column group)
-template <typename T, typename F>
-__device__ __forceinline__ void block_product(const T (*M)[NB + 1], const T (*X)[kXs], int nq4, F &&out)
+template <typename T, int BS, int NT, typename F>
+__device__ __forceinline__ void block_product(const T (*M)[BS + 1], const T (*X)[kXs], int nq4, F &&out)
 {
-    for (int it = threadIdx.x; it < NB * nq4; it += 256)
+    for (int it = threadIdx.x; it < BS * nq4; it += NT)
     {
         const int r = it / nq4, q = 4 * (it - r * nq4);
         T s0 = T(0), s1 = T(0), s2 = T(0), s3 = T(0);
 #pragma unroll 16
-        for (int c = 0; c < NB; ++c)
+        for (int c = 0; c < BS; ++c)
         {
             const T m = M[r][c];
             s0 = fma(m, X[c][q], s0);
@@ -920,16 +922,16 @@ __device__ __forceinline__ void block_product(const T (*M)[NB + 1], const T (*X)
     }
 }
 
-template <typename T, bool LOWER>
-__global__ void __launch_bounds__(256) trsv_flow_kernel(const T *__restrict__ A, int lda, int n, T *__restrict__ B,
+template <typename T, bool LOWER, int BS, int NT>
+__global__ void __launch_bounds__(NT) trsv_flow_kernel(const T *__restrict__ A, int lda, int n, T *__restrict__ B,
                                                         int nrhs, const float *__restrict__ dinv,
                                                         unsigned *__restrict__ sync, unsigned epoch,
                                                         int *__restrict__ err)
 {
     main_stream_priority<3>();
-    __shared__ T Ls[NB][NB + 1];
-    __shared__ __align__(16) T xs[NB][kXs];
-    __shared__ __align__(16) T acc[NB][kXs];
+    __shared__ T Ls[BS][BS + 1];
+    __shared__ __align__(16) T xs[BS][kXs];
+    __shared__ __align__(16) T acc[BS][kXs];
     __shared__ int s_t, s_ok;
     const int tid = threadIdx.x;
     if (tid == 0)
@@ -939,12 +941,12 @@ __global__ void __launch_bounds__(256) trsv_flow_kernel(const T *__restrict__ A,
     }
     __syncthreads();
     unsigned *flag = sync + kFlowHeader;
-    const int nblk = (n + NB - 1) / NB;
+    const int nblk = (n + BS - 1) / BS;
     const int t = s_t; // solve order
     const int ib = LOWER ? t : nblk - 1 - t;
-    const int r0 = ib * NB, nr = min(NB, n - r0);
+    const int r0 = ib * BS, nr = min(BS, n - r0);
     const int nq4 = (nrhs + 3) / 4, nqp = 4 * nq4;
-    constexpr int kPer = NB * NB / 256;
+    constexpr int kPer = BS * BS / NT;
     // the diagonal block's operator (inverse for FP32, the factor block for FP64) and the
     // first off-diagonal block go to registers now; later blocks are fetched one ahead and
     // staged in LDS before the wait for their right-hand side
@@ -952,18 +954,18 @@ __global__ void __launch_bounds__(256) trsv_flow_kernel(const T *__restrict__ A,
 #pragma unroll
     for (int k = 0; k < kPer; ++k)
     {
-        const int idx = tid + 256 * k, r = idx / NB, c = idx % NB;
+        const int idx = tid + NT * k, r = idx / BS, c = idx % BS;
         if constexpr (sizeof(T) == 4)
-            dreg[k] = dinv[(size_t)(2 * ib + (LOWER ? 0 : 1)) * NB * NB + idx];
+            dreg[k] = dinv[(size_t)(2 * ib + (LOWER ? 0 : 1)) * BS * BS + idx];
         else
             dreg[k] = (r < nr && c < nr) ? A[(size_t)(r0 + r) * lda + r0 + c] : (r == c ? T(1) : T(0));
     }
     auto fetch_l = [&](int k) {
-        const int c0 = (LOWER ? k : nblk - 1 - k) * NB, nc = min(NB, n - c0);
+        const int c0 = (LOWER ? k : nblk - 1 - k) * BS, nc = min(BS, n - c0);
 #pragma unroll
         for (int kk = 0; kk < kPer; ++kk)
         {
-            const int idx = tid + 256 * kk, r = idx / NB, c = idx % NB;
+            const int idx = tid + NT * kk, r = idx / BS, c = idx % BS;
             lreg[kk] = (r < nr && c < nc) ? A[(size_t)(r0 + r) * lda + c0 + c] : T(0);
         }
     };
@@ -971,11 +973,11 @@ __global__ void __launch_bounds__(256) trsv_flow_kernel(const T *__restrict__ A,
 #pragma unroll
         for (int kk = 0; kk < kPer; ++kk)
         {
-            const int idx = tid + 256 * kk;
-            Ls[idx / NB][idx % NB] = reg[kk];
+            const int idx = tid + NT * kk;
+            Ls[idx / BS][idx % BS] = reg[kk];
         }
     };
-    for (int idx = tid; idx < NB * nqp; idx += 256)
+    for (int idx = tid; idx < BS * nqp; idx += NT)
     {
         const int r = idx / nqp, q = idx - r * nqp;
         acc[r][q] = (r < nr && q < nrhs) ? B[(size_t)(r0 + r) * nrhs + q] : T(0);
@@ -989,17 +991,17 @@ __global__ void __launch_bounds__(256) trsv_flow_kernel(const T *__restrict__ A,
     }
     for (int k = 0; k < t; ++k)
     {
-        const int c0 = (LOWER ? k : nblk - 1 - k) * NB, nc = min(NB, n - c0);
+        const int c0 = (LOWER ? k : nblk - 1 - k) * BS, nc = min(BS, n - c0);
         if (tid == 0 && !flow_wait(&flag[k], epoch, err))
             s_ok = 0;
         __syncthreads();
-        for (int idx = tid; idx < NB * nqp; idx += 256)
+        for (int idx = tid; idx < BS * nqp; idx += NT)
         {
             const int r = idx / nqp, q = idx - r * nqp;
             xs[r][q] = (r < nc && q < nrhs) ? xload(&B[(size_t)(c0 + r) * nrhs + q]) : T(0);
         }
         __syncthreads();
-        block_product<T>(Ls, xs, nq4, [&](int r, int q, T a0, T a1, T a2, T a3) {
+        block_product<T, BS, NT>(Ls, xs, nq4, [&](int r, int q, T a0, T a1, T a2, T a3) {
             acc[r][q] -= a0;
             acc[r][q + 1] -= a1;
             acc[r][q + 2] -= a2;
@@ -1018,7 +1020,7 @@ __global__ void __launch_bounds__(256) trsv_flow_kernel(const T *__restrict__ A,
     const T poison = s_ok ? T(0) : T(NAN);
     if constexpr (sizeof(T) == 4)
     {
-        block_product<T>(Ls, acc, nq4, [&](int r, int q, T a0, T a1, T a2, T a3) {
+        block_product<T, BS, NT>(Ls, acc, nq4, [&](int r, int q, T a0, T a1, T a2, T a3) {
             if (r < nr)
             {
                 const T v[4] = {a0, a1, a2, a3};
@@ -1032,6 +1034,7 @@ __global__ void __launch_bounds__(256) trsv_flow_kernel(const T *__restrict__ A,
         // wave wv holds the right-hand sides wv, wv + 4, ... (at most kMaxRhs / 4) in
         // registers and advances them together; the current row's value is broadcast with
         // readlane
+        static_assert(BS == 64 && NT == 256, "FP64 substitution: one row per lane, four waves");
         const int lane = tid & 63, wv = tid >> 6;
         constexpr int QW = kMaxRhs / 4;
         T x[QW];
@@ -1042,9 +1045,9 @@ __global__ void __launch_bounds__(256) trsv_flow_kernel(const T *__restrict__ A,
             x[j] = q < nrhs ? acc[lane][q] : T(0);
         }
         const T lrow_diag = Ls[lane][lane];
-        for (int kk = 0; kk < NB; ++kk)
+        for (int kk = 0; kk < BS; ++kk)
         {
-            const int k = LOWER ? kk : NB - 1 - kk;
+            const int k = LOWER ? kk : BS - 1 - kk;
             const T lk = Ls[lane][k];
 #pragma unroll
             for (int j = 0; j < QW; ++j)
@@ -1528,11 +1531,11 @@ template <typename T> int finish_factor(pvl_lu *h, bool interchanged)
         // Only solves whose right-hand sides exceed one workgroup's LDS use the inverses.
         if ((size_t)n * kMaxRhs * sizeof(float) > (size_t)kTrsvWgBytes)
         {
-            const int nblk = (n + NB - 1) / NB;
+            const int nblk = (n + kFlowBs32 - 1) / kFlowBs32;
             if (!h->dinv)
-                PVL_HIP(hipMalloc(&h->dinv, sizeof(float) * 2 * (size_t)nblk * NB * NB));
+                PVL_HIP(hipMalloc(&h->dinv, sizeof(float) * 2 * (size_t)nblk * kFlowBs32 * kFlowBs32));
             ProfScope prof("lu_diag_inverse_f32");
-            diag_inverse_kernel<<<dim3(nblk, 2), 256, 0, stream()>>>((const float *)h->lu, n, n, h->dinv);
+            diag_inverse_kernel<kFlowBs32><<<dim3(nblk, 2), 256, 0, stream()>>>((const float *)h->lu, n, n, h->dinv);
             PVL_HIP(hipGetLastError());
             h->dinv_valid = true;
         }
@@ -1558,7 +1561,7 @@ template <typename T> int finish_factor(pvl_lu *h, bool interchanged)
 
 template <typename T> int solve_impl(pvl_lu *h, T *B, int nrhs)
 {
-    const int n = h->n, nblk = (n + NB - 1) / NB;
+    const int n = h->n;
     if (int st = ensure_ws())
         return st;
     T *work = (T *)h->work;
@@ -1595,16 +1598,19 @@ template <typename T> int solve_impl(pvl_lu *h, T *B, int nrhs)
             ProfScope prof(sizeof(T) == 8 ? "lu_trsv_f64" : "lu_trsv_f32");
             const T *Ap = (const T *)h->lu;
             flow_reset_kernel<<<1, 64, 0, stream()>>>(h->flow, g_ws.err + 5);
+            // FP32: 128-row blocks (half the dependent steps of 64), 512 threads
+            constexpr int BS = sizeof(T) == 4 ? kFlowBs32 : NB, NT = sizeof(T) == 4 ? 512 : 256;
+            const int nb = (n + BS - 1) / BS;
             for (int pass = 0; pass < 2; ++pass)
             {
                 if (++h->epoch == 0)
                     h->epoch = 1;
                 if (pass == 0)
-                    trsv_flow_kernel<T, true><<<nblk, 256, 0, stream()>>>(Ap, n, n, work, nq, h->dinv, h->flow,
-                                                                         h->epoch, g_ws.err + 5);
+                    trsv_flow_kernel<T, true, BS, NT><<<nb, NT, 0, stream()>>>(Ap, n, n, work, nq, h->dinv, h->flow,
+                                                                              h->epoch, g_ws.err + 5);
                 else
-                    trsv_flow_kernel<T, false><<<nblk, 256, 0, stream()>>>(Ap, n, n, work, nq, h->dinv, h->flow,
-                                                                          h->epoch, g_ws.err + 5);
+                    trsv_flow_kernel<T, false, BS, NT><<<nb, NT, 0, stream()>>>(Ap, n, n, work, nq, h->dinv, h->flow,
+                                                                               h->epoch, g_ws.err + 5);
             }
             PVL_HIP(hipGetLastError());
         }
