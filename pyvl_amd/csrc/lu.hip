@@ -13,8 +13,9 @@
 //   laswp_kernel   applies the panel's interchanges to the other columns.
 //   trsm_kernel    U12 = L11^-1 A12 (one lane per column in registers, L11 in LDS).
 //   gemm_kernel    A22 -= L21 U12 on MFMA (v_mfma_f64_16x16x4f64 / v_mfma_f32_16x16x4f32).
-// Solves run one launch per 64-row block (update of the later blocks, then a wave-level
-// substitution with the next diagonal block).
+// Solves: one workgroup for systems whose right-hand sides fit its LDS (trsv_wg_kernel),
+// else one launch per triangle with a workgroup per block row that waits on the flags of
+// the blocks before it (trsv_flow_kernel).
 #include "pvl_internal.hpp"
 #include "lu_common.hpp"
 
@@ -856,7 +857,7 @@ __global__ void __launch_bounds__(256) diag_inverse_kernel(const float *__restri
 }
 
 // Block triangular solve on B (n x nrhs, row-major), in place, in ONE launch of one
-// workgroup per 64-row block.  LOWER: unit lower L, forward order; else upper U, backward
+// workgroup per BS-row block (64 rows for FP64, 128 for FP32).  LOWER: unit lower L, forward order; else upper U, backward
 // order.  Workgroups take solve-order indices from a ticket counter, so every smaller index
 // belongs to a workgroup already running; workgroup t subtracts the contributions of blocks
 // 0 .. t-1 in that order as their flags (sync[kFlowHeader + k] == epoch) appear, each block
@@ -1514,7 +1515,7 @@ struct pvl_lu
     int *perm = nullptr;   // device, n: row i of P B is row perm[i] of B
     void *work = nullptr;  // n * kMaxRhs (T) staging for permuted right-hand sides
     int pivoting = PVL_PIVOT_PARTIAL;
-    float *dinv = nullptr; // FP32: inverses of the 64 x 64 diagonal blocks of L and U
+    float *dinv = nullptr; // FP32: inverses of the 128 x 128 diagonal blocks of L and U
     bool dinv_valid = false;
     unsigned *flow = nullptr; // tickets and block flags of the single-launch solves
     unsigned epoch = 0;       // flag value of the latest solve launch
