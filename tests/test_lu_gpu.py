@@ -85,6 +85,21 @@ def test_block_solve_paths(prec, n, nrhs):
     assert np.array_equal(x2, x)
 
 
+def test_block_solve_after_refactor():
+    """One FP32 handle factored twice (different matrices): the second solve uses the second
+    factorisation's diagonal-block inverses, not the first's."""
+    n, nrhs = 1500, 16
+    rng = np.random.default_rng(5)
+    h = LA.LU(n, LA.FP32)
+    for shift in (2.0, 5.0):
+        a = rng.normal(size=(n, n)) + shift * n ** 0.5 * np.eye(n)
+        b = rng.normal(size=(n, nrhs))
+        h.factor(a)
+        x = h.solve(b.astype(np.float32)).astype(np.float64)
+        x_ref = _solve_with_factors(h.factors(), h.pivots(), b)
+        assert _rel(x, x_ref) < 1e-5
+
+
 def test_scipy_call_site_dropin():
     """lu_factor(A, overwrite_a=True) then lu_solve(decomp, b, overwrite_b=True), as
     python/pyvl/solver.py:223-227 calls them."""
