@@ -621,9 +621,9 @@ __global__ void __launch_bounds__(256) trsm_kernel(T *__restrict__ A, int lda, i
 // segment (one panel's L21 U12) is summed from zero and subtracted from C once (as dgemm's
 // C - A B: one rounding at |C|'s magnitude per panel, not one per k on diagonally dominant
 // updates); the C tile stays in registers from the start to the end, across segments.  K is
-// staged through LDS in chunks of 32, software-pipelined: the next chunk's operands are
-// loaded into registers while the current chunk feeds the MFMAs, so global latency overlaps
-// the matrix work (FP64: 164 VGPRs, three workgroups per CU; FP32: four).
+// staged through LDS in chunks (16 wide for FP64, 32 for FP32), software-pipelined: the next
+// chunk's operands are loaded into registers while the current chunk feeds the MFMAs, so
+// global latency overlaps the matrix work (at most 128 VGPRs: four workgroups per CU).
 // Tile order (1-D grid): consecutive workgroups go round-robin to the 8 XCDs, each with its
 // own L2, so workgroup g works for XCD g % 8, which owns a stripe of cpx column tiles and
 // sweeps it row tile by row tile: its U12 stripe stays in its L2 and each L21 tile is
@@ -644,11 +644,11 @@ template <> struct Vec16<float>
 template <typename T> using V16 = typename Vec16<T>::type;
 
 template <typename T, bool AL>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 8 ? 3 : 4))) gemm_kernel(T *__restrict__ A, int lda, int row0, int m2, int col0, int n2,
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) gemm_kernel(T *__restrict__ A, int lda, int row0, int m2, int col0, int n2,
                                                    int kk0, int K, int cpx)
 {
     main_stream_priority<2>();
-    constexpr int BM = 64, BN = 64, KC = 32;
+    constexpr int BM = 64, BN = 64, KC = sizeof(T) == 8 ? 16 : 32;
     // Row strides 36 and 80 elements: the operand reads (16 rows x 4 k, 4 k x 16 columns per
     // wave) then fall on distinct LDS banks apart from the two passes a 64-lane read needs.
     __shared__ __align__(16) T As[BM][KC + 4];
