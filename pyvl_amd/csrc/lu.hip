@@ -1336,7 +1336,7 @@ int ensure_ws()
     return PVL_OK;
 }
 
-template <typename T> int panel_rows_max() { return sizeof(T) == 8 ? 256 : 512; }
+template <typename T> int panel_rows_max() { return 256; }
 
 template <typename T> int factor_impl(int n, T *A, int *piv, int *info_out, bool pivoting, bool *would_pivot = nullptr)
 {
