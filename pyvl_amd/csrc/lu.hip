@@ -96,7 +96,9 @@ __device__ __forceinline__ void xstore_i(int *p, int v)
 // Grid barrier on a monotonically increasing counter; bounded: on timeout sets *err.
 __device__ __forceinline__ void grid_barrier(unsigned *counter, unsigned target, int *err)
 {
-    __threadfence();
+    // every wave's exchange stores (device-coherent) complete; thread 0's release covers the
+    // workgroup
+    __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
     if (threadIdx.x == 0)
     {
