@@ -30,7 +30,8 @@ constexpr unsigned kSpin = 20000000u;
 
 __device__ __forceinline__ void gbar(unsigned *counter, unsigned target, int *err)
 {
-    __threadfence();
+    // every wave's stores complete; thread 0's release (an L2 write-back) covers the workgroup
+    __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
     if (threadIdx.x == 0)
     {
