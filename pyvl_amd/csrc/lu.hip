@@ -1072,7 +1072,9 @@ __global__ void __launch_bounds__(NT) trsv_flow_kernel(const T *__restrict__ A, 
                 xstore(&B[(size_t)(r0 + lane) * nrhs + q], x[j] + poison);
         }
     }
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    // every wave's solution stores (device-coherent) complete; thread 0's release store of
+    // the flag covers the workgroup
+    __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
     if (tid == 0)
         __hip_atomic_store(&flag[t], epoch, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
