@@ -542,12 +542,11 @@ __global__ void __launch_bounds__(kNopivThreads) __attribute__((amdgpu_waves_per
 // Row interchanges of one panel applied to every column outside it.
 template <typename T>
 __global__ void __launch_bounds__(256) laswp_kernel(T *__restrict__ A, int lda, int n, int k, int jb,
-                                                   const int *__restrict__ piv, int lo, int hi, int inside)
+                                                   const int *__restrict__ piv)
 {
     main_stream_priority();
-    // columns [lo, hi) (inside) or the others (!inside); never the panel's own
     const int c = blockIdx.x * 256 + threadIdx.x;
-    if (c >= n || (c >= k && c < k + jb) || ((c >= lo && c < hi) != (inside != 0)))
+    if (c >= n || (c >= k && c < k + jb))
         return;
     for (int i = 0; i < jb; ++i)
     {
@@ -648,11 +647,9 @@ template <typename T> using V16 = typename Vec16<T>::type;
 
 template <typename T, bool AL>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) gemm_kernel(T *__restrict__ A, int lda, int row0, int m2, int col0, int n2,
-                                                   int kk0, int K, int cpx, int prio)
+                                                   int kk0, int K, int cpx)
 {
-    // the look-ahead part of a trailing update yields issue slots to the panel's waves
-    if (prio)
-        main_stream_priority<2>();
+    main_stream_priority<2>();
     constexpr int BM = 64, BN = 64, KC = sizeof(T) == 8 ? 16 : 32;
     // Row strides 36 and 80 elements: the operand reads (16 rows x 4 k, 4 k x 16 columns per
     // wave) then fall on distinct LDS banks apart from the two passes a 64-lane read needs.
@@ -1345,27 +1342,6 @@ int ensure_ws()
 
 template <typename T> int panel_rows_max() { return 256; }
 
-// Second stream and events of the factorisation's look-ahead (created once, default
-// priority): the bulk of an outer block's trailing update runs there while the library
-// stream factors the next block's panels.
-struct LookAhead
-{
-    hipStream_t s = nullptr;
-    hipEvent_t ev_row = nullptr, ev_b = nullptr;
-};
-int look_ahead(LookAhead *out)
-{
-    static LookAhead g;
-    if (!g.s)
-    {
-        PVL_HIP(hipStreamCreateWithFlags(&g.s, hipStreamNonBlocking));
-        PVL_HIP(hipEventCreateWithFlags(&g.ev_row, hipEventDisableTiming));
-        PVL_HIP(hipEventCreateWithFlags(&g.ev_b, hipEventDisableTiming));
-    }
-    *out = g;
-    return PVL_OK;
-}
-
 template <typename T> int factor_impl(int n, T *A, int *piv, int *info_out, bool pivoting, bool *would_pivot = nullptr)
 {
     if (int st = ensure_ws())
@@ -1394,22 +1370,20 @@ template <typename T> int factor_impl(int n, T *A, int *piv, int *info_out, bool
     // right) and the block's remaining columns below it.  For n <= 2048 the outer block is
     // one panel and the updates reduce to the usual one per panel.
     const int nbo = n > 2048 ? 4 * nbf : nbf;
-    hipStream_t gs = stream(); // stream of the next gemm() launch
     auto gemm = [&](int row0, int m2, int col0, int n2, int kk0, int K) -> int {
         if (m2 <= 0 || n2 <= 0 || K <= 0)
             return PVL_OK;
         const int nt = (n2 + 63) / 64, cpx = (nt + kXcds - 1) / kXcds;
         const int mt = (m2 + 63) / 64;
-        ProfScope prof(sizeof(T) == 8 ? "lu_gemm_f64" : "lu_gemm_f32", 0.0, gs);
+        ProfScope prof(sizeof(T) == 8 ? "lu_gemm_f64" : "lu_gemm_f32");
         bool vec = false;
         if constexpr (sizeof(T) == 4)
             vec = n % 4 == 0;
         if (vec)
-            gemm_kernel<float, true><<<kXcds * mt * cpx, 256, 0, gs>>>((float *)A, n, row0, m2, col0, n2, kk0, K,
-                                                                       cpx, gs == stream() ? 1 : 0);
+            gemm_kernel<float, true><<<kXcds * mt * cpx, 256, 0, stream()>>>((float *)A, n, row0, m2, col0, n2, kk0,
+                                                                             K, cpx);
         else
-            gemm_kernel<T, false><<<kXcds * mt * cpx, 256, 0, gs>>>(A, n, row0, m2, col0, n2, kk0, K, cpx,
-                                                                    gs == stream() ? 1 : 0);
+            gemm_kernel<T, false><<<kXcds * mt * cpx, 256, 0, stream()>>>(A, n, row0, m2, col0, n2, kk0, K, cpx);
         PVL_HIP(hipGetLastError());
         return PVL_OK;
     };
@@ -1426,16 +1400,6 @@ template <typename T> int factor_impl(int n, T *A, int *piv, int *info_out, bool
         PVL_HIP(hipGetLastError());
         return PVL_OK;
     };
-    // Look-ahead (FP32 partial pivoting with outer blocks): see the trailing update below.
-    // FP64 does without: its panels slow down beside the concurrent update about as much as
-    // the overlap gains (16k: panels 111 -> 171 ms, factorisation 207 -> 210 ms), where FP32
-    // gains (165 -> 156 ms).
-    const bool ahead = pivoting && nbo > nbf && sizeof(T) == 4;
-    LookAhead la;
-    if (ahead)
-        if (int st = look_ahead(&la))
-            return st;
-    bool pending = false; // an update of the columns right of the current block is in flight
     for (int ko = 0; ko < n; ko += nbo)
     {
     const int kend = std::min(n, ko + nbo);
@@ -1485,10 +1449,8 @@ template <typename T> int factor_impl(int n, T *A, int *piv, int *info_out, bool
         // info / err readback is deferred to the end (one sync per factorisation)
         if (pivoting)
         {
-            // with look-ahead only the block's own columns now (the others after the block)
             ProfScope prof(sizeof(T) == 8 ? "lu_laswp_f64" : "lu_laswp_f32");
-            laswp_kernel<T><<<(n + 255) / 256, 256, 0, stream()>>>(A, n, n, k, jb, piv, ahead ? ko : 0,
-                                                                  ahead ? kend : 0, ahead ? 1 : 0);
+            laswp_kernel<T><<<(n + 255) / 256, 256, 0, stream()>>>(A, n, n, k, jb, piv);
             PVL_HIP(hipGetLastError());
         }
         if (pivoting)
@@ -1513,19 +1475,6 @@ template <typename T> int factor_impl(int n, T *A, int *piv, int *info_out, bool
     }
     if (pivoting)
     {
-        if (ahead)
-        {
-            // the previous block's update of the columns right of this block has finished
-            // before they (and the columns left of the block) take this block's interchanges
-            if (pending)
-                PVL_HIP(hipStreamWaitEvent(stream(), la.ev_b, 0));
-            pending = false;
-            ProfScope prof(sizeof(T) == 8 ? "lu_laswp_f64" : "lu_laswp_f32");
-            for (int k = ko; k < kend; k += nbf)
-                laswp_kernel<T><<<(n + 255) / 256, 256, 0, stream()>>>(A, n, n, k, std::min(nbf, n - k), piv, ko,
-                                                                      kend, 0);
-            PVL_HIP(hipGetLastError());
-        }
         // block row right of the outer block: U12 = L11^-1 A12 panel by panel
         for (int k = ko; k < kend; k += nbf)
         {
@@ -1536,27 +1485,10 @@ template <typename T> int factor_impl(int n, T *A, int *piv, int *info_out, bool
                 return st;
         }
     }
-    // the trailing matrix, once per outer block.  Look-ahead: the next block's columns on the
-    // library stream, the rest on the look-ahead stream, overlapping the next block's panels
-    if (ahead && n - kend > nbo)
-    {
-        PVL_HIP(hipEventRecord(la.ev_row, stream()));
-        if (int st = gemm(kend, n - kend, kend, nbo, ko, kend - ko))
-            return st;
-        PVL_HIP(hipStreamWaitEvent(la.s, la.ev_row, 0));
-        gs = la.s;
-        const int st = gemm(kend, n - kend, kend + nbo, n - kend - nbo, ko, kend - ko);
-        gs = stream();
-        if (st)
-            return st;
-        PVL_HIP(hipEventRecord(la.ev_b, la.s));
-        pending = true;
-    }
-    else if (int st = gemm(kend, n - kend, kend, n - kend, ko, kend - ko))
+    // the trailing matrix, once per outer block
+    if (int st = gemm(kend, n - kend, kend, n - kend, ko, kend - ko))
         return st;
     }
-    if (pending)
-        PVL_HIP(hipStreamWaitEvent(stream(), la.ev_b, 0));
     status_to_host_kernel<<<1, 64, 0, stream()>>>(status, g_status_host_dev);
     PVL_HIP(hipGetLastError());
     PVL_HIP(hipStreamSynchronize(stream()));
