@@ -5,10 +5,12 @@ solver the reference calls).
 Follows python/pyvl/wake_models.py:35-223 and python/pyvl/solver.py:175-239 line by line,
 in the reference layout: wake_positions (n_shed, 2, R+1, 3), circulation (n_shed, R), the
 full ring mesh of _create_wake_mesh with every (also unfilled) row evaluated each step, and
-the per-row convection loop.  Not pinned by any reference test beyond the circulation
-shift (test/pytests/test_wake_models.py:75-77,155-157): the positions and corrected systems
-rest on this restatement ("parity unpinned" in the sense of the oracle's README) and are
-checked here only against the GPU path and the shift/difference invariants.
+the per-row convection loop.  Pinned by the reference tests' circulation shift
+(test/pytests/test_wake_models.py:75-77,155-157) and by what the mathematics fixes
+(tests/test_wake_oracle_pins.py): the free-stream convection closed form (bit-exact), the
+convection velocity by a second induction route, the consistency of the implicit columns
+with the next update's newest-row circulation, the row split of the corrections, and
+mirror symmetry; each check also rejects a copy with a planted mistake.
 """
 
 from __future__ import annotations
