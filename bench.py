@@ -36,6 +36,7 @@ sys.path.insert(0, str(ROOT))
 import numpy as np  # noqa: E402
 
 PEAK_FP64_TFLOPS = 78.6  # MI355X FP64 (vector and MFMA)
+PEAK_FP32_TFLOPS = 157.3  # MI355X FP32 (vector and MFMA)
 # HBM bytes per launch of the full-wake self-induction kernel: rocprofv3 FETCH_SIZE and
 # WRITE_SIZE in two runs of this script (--copy-calibration), each scaled by what the same
 # counter reads for the script's own 1 GiB copy (FETCH x 2.0, WRITE x 1.0); the commands and
@@ -112,14 +113,14 @@ def roofline_from(prof) -> dict:
     avg_ms = ms / max(cnt, 1)
     achieved = flops / (ms * 1e-3) / 1e12 if ms > 0 else 0.0
     return {
-        "bound": "mfma",
+        "bound": "valu_fp64",
         "achieved": round(achieved, 3),
         "peak": PEAK_FP64_TFLOPS,
         "unit": "TFLOP/s",
         "frac": round(achieved / PEAK_FP64_TFLOPS, 4),
         "traffic": SELF_VELOCITY_HBM_BYTES,
         "kernel": "point_velocity_kernel<FP64, wake-self> (free-wake self-induction)",
-        "note": "FP64 vector-ALU bound (MI355X FP64 vector peak = FP64 MFMA peak = 78.6 TFLOP/s); "
+        "note": "FP64 vector-ALU bound (peak: MI355X FP64 vector rate, 78.6 TFLOP/s, equal to its FP64 MFMA rate); "
                 f"algorithmic {FLOP_PER_PAIR + FLOP_WEIGHT:.0f} FLOP per (segment, point) pair, "
                 f"{pairs / max(cnt, 1):.4g} pairs per launch, avg {avg_ms:.3f} ms over {cnt} launches "
                 "(HIP events on the side stream the kernel is launched on, around this kernel only, over "
@@ -127,125 +128,83 @@ def roofline_from(prof) -> dict:
     }
 
 
-def cpu_free_wake_step(gpu_run) -> dict:
-    """One full-wake free-wake step on the host, oracle C restatement (reference layout)."""
-    import scipy.linalg as la
-
+def _oracle_from(sol):
+    """The oracle's bound mesh and reference-layout wake, loaded with the GPU solver's state."""
     from oracle import oracle as O
     from oracle.wake_oracle import OracleWake
 
-    sol = gpu_run["sol"]
-    w = sol.wake
-    pos0 = sol.p0
     el_counts, flat = sol.mesh.to_element_connectivity()
     offs = np.concatenate([[0], np.cumsum(el_counts)]).astype(np.int64)
     elements = [flat[offs[i]:offs[i + 1]].tolist() for i in range(len(el_counts))]
-    bound = O.mesh_from_elements(pos0.shape[0], elements)
+    bound = O.mesh_from_elements(sol.p0.shape[0], elements)
+    w = sol.wake
     ow = OracleWake(w.bordering_nodes, w.shedding_lines, w.adjacent_surfaces, C3["tol"], 0.0, C3["rows"])
     wp, wc = w.state()
     ow.wake_positions[...] = wp
     ow.circulation[...] = wc
     ow.step_count = w.step_count
     ow.current_time = w.current_time
+    return bound, ow
+
+
+def _step_errors(sol, gammas_cpu, ow, n_steps) -> dict:
+    """The GPU solver's next n_steps from the same state, against the host's."""
+    err_g = 0.0
+    for k in range(n_steps):
+        g = sol.step().numpy()
+        err_g = max(err_g, float(np.linalg.norm(g - gammas_cpu[k]) / np.linalg.norm(gammas_cpu[k])))
+    wp, wc = sol.wake.state()
+    return {
+        "rel_err_circulation": err_g,
+        "rel_err_wake_positions": float(np.max(np.abs(wp - ow.wake_positions)) / np.max(np.abs(ow.wake_positions))),
+        "rel_err_wake_circulation": float(np.max(np.abs(wc - ow.circulation)) / np.max(np.abs(ow.circulation))),
+    }
+
+
+def cpu_free_wake_step(gpu_run) -> dict:
+    """One full-wake free-wake step on the host, oracle C restatement (reference layout),
+    from the GPU's state; the GPU's own next step is then compared with it."""
+    from oracle import wake_oracle as WO
+
+    sol = gpu_run["sol"]
+    bound, ow = _oracle_from(sol)
     it = sol.iteration
     vinf = np.array([1.0, 0.0, 0.0])
     t0 = time.perf_counter()
-    # one run_solver iteration with free-wake convection
-    t = it * sol.dt
-    pos = sol.frame.positions(pos0, t)
-    vel = sol.frame.velocities(pos0, t)
-    cps = O.surface_center(bound, pos)
-    cpv = O.surface_center(bound, vel)
-    nrm = O.surface_normal(bound, pos)
-    b = np.vecdot(nrm, -(np.full_like(cps, vinf) - cpv), axis=1)
-    a = O.induction_matrix3(bound, pos, cps, nrm, C3["tol"])
-    ow.apply_corrections(cps, nrm, a, b)
-    gamma = la.lu_solve(la.lu_factor(a), b)
-    cr = min(ow.step_count, ow.line_rows)
-    old = ow.wake_positions[:, :, :cr, :].copy()
-    flat_pts = np.ascontiguousarray(old.reshape(-1, 3))
-    v = np.full_like(flat_pts, vinf)
-    v += O.point_velocity(bound, pos, gamma, flat_pts, C3["tol"])
-    v += O.point_velocity(ow.wake_mesh, ow.wake_positions.reshape(-1, 3), ow.circulation.reshape(-1), flat_pts,
-                          C3["tol"])
-    ow.wake_positions[:, :, 1:cr + 1, :] = old + v.reshape(old.shape) * sol.dt
+    gamma, _, _ = WO.step(it, sol.dt, bound, sol.p0, sol.frame, vinf, C3["tol"], ow, free=True)
     elapsed = time.perf_counter() - t0
+    errs = _step_errors(sol, [gamma], ow, 1)
     return {
         "value": round(1.0 / elapsed, 4),
         "unit": "steps/s",
         "cores": os.cpu_count(),
         "kind": "port",
         "sample": f"1 full-wake free-wake step (step {it + 1}, 25,600 wake panels, reference layout: "
-                  f"{flat_pts.shape[0]} wake nodes x {ow.wake_mesh.n_lines} wake lines) from the GPU's wake "
-                  f"state; oracle C restatement, gcc -O3 no FMA, OpenMP {os.cpu_count()} threads; "
+                  f"{ow.wake_positions[:, :, :-1].size // 3} wake nodes x {ow.wake_mesh.n_lines} wake lines) from the "
+                  f"GPU's wake state; oracle C restatement, gcc -O3 no FMA, OpenMP {os.cpu_count()} threads; "
                   f"LU = scipy LAPACK; {elapsed:.2f} s",
+        "gpu_vs_cpu": errs,
     }
 
 
 def cpu_reference_step(gpu_ref_run, n_steps: int) -> dict:
-    """Reference-semantics steps on the host with the reference's own C core (oracle/_ref)."""
-    import scipy.linalg as la
-
+    """Reference-semantics steps on the host with the reference's own C core (oracle/_ref),
+    from the GPU's state; the GPU's own next steps are then compared with them."""
     from oracle import oracle as O
-    from oracle.wake_oracle import OracleWake
+    from oracle import wake_oracle as WO
 
     if O.load_ref() is None:
         return {"skipped": "oracle/_ref not built"}
     sol = gpu_ref_run["sol"]
-    w = sol.wake
-    pos0 = sol.p0
-    el_counts, flat = sol.mesh.to_element_connectivity()
-    offs = np.concatenate([[0], np.cumsum(el_counts)]).astype(np.int64)
-    elements = [flat[offs[i]:offs[i + 1]].tolist() for i in range(len(el_counts))]
-    bound = O.mesh_from_elements(pos0.shape[0], elements)
-    ow = OracleWake(w.bordering_nodes, w.shedding_lines, w.adjacent_surfaces, C3["tol"], 0.0, C3["rows"])
-    wp, wc = w.state()
-    ow.wake_positions[...] = wp
-    ow.circulation[...] = wc
-    ow.step_count = w.step_count
-    ow.current_time = w.current_time
+    bound, ow = _oracle_from(sol)
     vinf = np.array([1.0, 0.0, 0.0])
     it0 = sol.iteration
+    gammas = []
     t0 = time.perf_counter()
     for k in range(n_steps):
-        it = it0 + k
-        t = it * sol.dt
-        pos = sol.frame.positions(pos0, t)
-        vel = sol.frame.velocities(pos0, t)
-        cps = O.ref_surface_center(bound, pos)
-        cpv = O.ref_surface_center(bound, vel)
-        nrm = O.ref_surface_normal(bound, pos)
-        b = np.vecdot(nrm, -(np.full_like(cps, vinf) - cpv), axis=1)
-        a = O.ref_induction_matrix3(bound, pos, cps, nrm, C3["tol"])
-        # wake corrections with the reference's own line-buffer path
-        n = ow.shedding_lines.size
-        ind = O.ref_induction_matrix3(ow.wake_mesh, ow.wake_positions.reshape(-1, 3), cps, nrm, C3["tol"])
-        ind = ind.reshape(cps.shape[0], n, ow.line_rows)
-        for i_line in range(n):
-            corr = ind[:, i_line, 0]
-            s1, s0 = ow.adjacent_surfaces[i_line, 1], ow.adjacent_surfaces[i_line, 0]
-            if s1 != 0x7FFFFFFF:
-                a[:, s1] -= corr
-            if s0 != 0x7FFFFFFF:
-                a[:, s0] += corr
-        b += np.vecdot(ind[:, :, 1:].reshape(cps.shape[0], -1), ow.circulation[:, 1:].reshape(-1)[None, :], axis=1)
-        gamma = la.lu_solve(la.lu_factor(a), b)
-        dt = t - ow.current_time
-        ow.current_time = t
-        cr = min(ow.step_count, ow.line_rows)
-        for i in reversed(range(cr)):
-            old = ow.wake_positions[:, :, i, :]
-            fl = np.array(old.reshape(-1, 3))
-            v = np.full_like(fl, vinf)
-            v += np.vecdot(O.ref_induction_matrix(bound, pos, fl, C3["tol"]), gamma[None, :, None], axis=1)
-            ow.wake_positions[:, :, i + 1, :] = old + v.reshape(old.shape) * dt
-        ow.circulation[:, 1:] = ow.circulation[:, :-1]
-        ow.circulation[:, 0] = 0
-        m = ow.adjacent_surfaces != 0x7FFFFFFF
-        ow.circulation[m[:, 1], 0] += gamma[ow.adjacent_surfaces[m[:, 1], 1]]
-        ow.circulation[m[:, 0], 0] -= gamma[ow.adjacent_surfaces[m[:, 0], 0]]
-        ow.step_count += 1
+        gammas.append(WO.reference_core_step(it0 + k, sol.dt, bound, sol.p0, sol.frame, vinf, C3["tol"], ow))
     elapsed = time.perf_counter() - t0
+    errs = _step_errors(sol, gammas, ow, n_steps)
     return {
         "value": round(n_steps / elapsed, 4),
         "unit": "steps/s",
@@ -255,6 +214,32 @@ def cpu_reference_step(gpu_ref_run, n_steps: int) -> dict:
                   "reference's own src/core C (compute_line_induction + line_induction_to_*_surface_induction "
                   "with its full line buffer, built unmodified into oracle/_ref), its Python step order restated "
                   f"in numpy, scipy LAPACK LU; {elapsed:.2f} s",
+        "gpu_vs_cpu": errs,
+    }
+
+
+def fp32_vs_fp64_step(sol32, sol64) -> dict:
+    """Error of the FP32-angle induction: the FP64 solver is loaded with the FP32 run's wake
+    state and both take the same next step (the FP64 step is itself within 1e-12 of the
+    oracle's); relative differences of circulation, wake positions and wake circulation."""
+    import ctypes as C
+
+    from pyvl_amd import _lib
+
+    d = sol32.wake.save()
+    pos = np.ascontiguousarray(d["wake_positions"], np.float64)
+    circ = np.ascontiguousarray(d["circulation"], np.float64)
+    _lib.call("pvl_wake_set_state", sol64.wake._h, pos.ctypes.data_as(C.c_void_p), circ.ctypes.data_as(C.c_void_p),
+              C.c_uint(int(d["step_count"])), C.c_double(float(d["current_time"])))
+    sol64.iteration = sol32.iteration
+    g64 = sol64.step().numpy()
+    g32 = sol32.step().numpy()
+    p64, c64 = sol64.wake.state()
+    p32, c32 = sol32.wake.state()
+    return {
+        "rel_err_circulation": float(np.linalg.norm(g32 - g64) / np.linalg.norm(g64)),
+        "rel_err_wake_positions": float(np.max(np.abs(p32 - p64)) / np.max(np.abs(p64))),
+        "rel_err_wake_circulation": float(np.max(np.abs(c32 - c64)) / np.max(np.abs(c64))),
     }
 
 
@@ -331,6 +316,49 @@ def run_c4(do_cpu: bool) -> dict:
     t32n = time.perf_counter() - t0
     x32n = mean_free(X32n.numpy())
     log(f"C4 FP32 LU without pivoting + FP64 IR (16 RHS): {t32n * 1e3:.1f} ms, {iters_n} refinement steps")
+    # per-kernel split of one more factorisation + solve of each kind (HIP events on every
+    # launch, untimed above); roofline of the whole solve and of its MFMA trailing update
+    kinds = {"fp64": (lu64, X64, 8, PEAK_FP64_TFLOPS, t64), "fp32_partial_pivoting_ir": (lu32, X32, 4, PEAK_FP32_TFLOPS, t32),
+             "fp32_no_pivoting_ir": (lu32n, X32n, 4, PEAK_FP32_TFLOPS, t32n)}
+    roof = {}
+    nrhs = 16
+    alg_flops = 2.0 / 3.0 * n ** 3 + 2.0 * n * n * nrhs
+    for name, (lu, X, es, peak, tw) in kinds.items():
+        sfx = "f64" if es == 8 else "f32"
+        lib.pvl_prof_reset()
+        lib.pvl_prof_enable(1)
+        if es == 8:
+            lu.factor(A)
+            _lib.call("pvl_memcpy_d2d", X.cptr, dB.cptr, C.c_size_t(dB.nbytes))
+            lu.solve(X)
+        else:
+            lu.factor(A)
+            LA.solve_refine(A, dB, lu, tol=1e-12, out=X)
+        _lib.call("pvl_synchronize")
+        split = {}
+        for k in ("lu_panel", "lu_gemm", "lu_trsm", "lu_laswp", "lu_trsv", "lu_diag_inverse"):
+            ms, cnt, work = prof_get(lib, f"{k}_{sfx}")
+            split[k] = {"ms": round(ms, 3), "launches": cnt}
+            if k == "lu_gemm":
+                gemm_tf = work / (ms * 1e-3) / 1e12 if ms > 0 else 0.0
+        for k in ("residual_f64",):
+            ms, cnt, _ = prof_get(lib, k)
+            if cnt:
+                split[k] = {"ms": round(ms, 3), "launches": cnt}
+        lib.pvl_prof_enable(0)
+        achieved = alg_flops / tw / 1e12
+        roof[name] = {
+            "bound": "mfma",
+            "achieved": round(achieved, 3),
+            "peak": peak,
+            "unit": "TFLOP/s",
+            "frac": round(achieved / peak, 4),
+            "note": f"whole solve: 2/3 n^3 + 2 n^2 nrhs = {alg_flops:.4g} flop (one factorisation, n = {n}, "
+                    f"{nrhs} right-hand sides) over its wall time; refinement's extra residuals not counted",
+            "trailing_update_gemm": {"achieved": round(gemm_tf, 3), "frac": round(gemm_tf / peak, 4)},
+            "kernel_ms": split,
+        }
+        log(f"C4 {name}: {split}; gemm {gemm_tf:.1f} TFLOP/s")
     bmax = np.max(np.abs(B), axis=0)
     res64 = float(np.max(np.max(np.abs(LA.residual(A, X64, dB).numpy()), axis=0) / bmax))
     rel = lambda a, b: float(np.max(np.linalg.norm(a - b, axis=0) / np.linalg.norm(b, axis=0)))  # noqa: E731
@@ -352,6 +380,7 @@ def run_c4(do_cpu: bool) -> dict:
         "deflation_c": c,
         "speedup_fp32ir_over_fp64": round(t64 / t32, 3),
         "speedup_fp32nopiv_ir_over_fp64": round(t64 / t32n, 3),
+        "roofline": roof,
     }
     if do_cpu:
         import scipy.linalg as la
@@ -474,6 +503,9 @@ def main() -> int:
     if not args.no_cpu:
         cpu = cpu_free_wake_step(head)
         log(f"CPU free wake: {cpu}")
+    if not args.headline_only:
+        extra["c3_free_wake_fp32_induction_vs_fp64"] = fp32_vs_fp64_step(f32["sol"], head["sol"])
+        log(f"C3 FP32 induction against FP64 from the same state: {extra['c3_free_wake_fp32_induction_vs_fp64']}")
     line = {
         "metric": "wake steps/s (1k bound+25k wake panels)",
         "value": round(head["steps_per_s"], 3),

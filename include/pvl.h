@@ -18,6 +18,13 @@
  *     except the work pvl_wake_begin_update starts on a second library stream (also default
  *     priority, ordered with events); functions that return device results are
  *     asynchronous unless stated otherwise, and pvl_synchronize() waits for both streams.
+ *   - Thread safety: entry points may be called from several host threads at once (the
+ *     reference releases the GIL around its compute, src/meshobject.c:462-468).  The
+ *     library's shared scratch, workspaces and streams are guarded by one process-wide
+ *     lock, so such calls run one after another, in the order they take the lock; the
+ *     host topology functions (pvl_mesh_from_elements, pvl_mesh_dual, pvl_wake_topology)
+ *     touch no shared state and run concurrently.  Handles (pvl_mesh, pvl_lu, pvl_wake,
+ *     pvl_mlp) must not be destroyed while another thread uses them.
  */
 #ifndef PVL_H
 #define PVL_H
@@ -59,7 +66,15 @@ enum pvl_precision
 const char *pvl_last_error(void);
 int pvl_init(int device);                 /* select device, create the library stream */
 int pvl_device_count(int *count);
+/* Waits for both library streams.  Also reports (PVL_ERR_HIP) an error that device work
+ * recorded after its entry point had returned: a single-launch triangular solve whose
+ * bounded wait timed out (its rows were written as NaN). */
 int pvl_synchronize(void);
+/* Releases the library's process-lifetime state (streams, events, pinned status words,
+ * scratch and solver workspaces) after waiting for its streams.  Call it before the HIP
+ * runtime is torn down (the Python shim registers it with atexit).  Handles created before
+ * it stay valid for destruction; any later call re-initialises the library on device 0. */
+int pvl_shutdown(void);
 int pvl_malloc(void **dev_ptr, size_t bytes);
 int pvl_free(void *dev_ptr);
 int pvl_memcpy_h2d(void *dev_dst, const void *host_src, size_t bytes);

@@ -100,8 +100,95 @@ class OracleWake:
         rhs_in[:] += np.vecdot(explicit, circ[None, :], axis=1)
 
 
-def step(it: int, dt: float, bound: O.Topo, p0, frame, vinf, tol, wake: OracleWake | None, closed=False):
-    """One iteration of python/pyvl/solver.py:175-239 for a single body."""
+def free_update(ow: OracleWake, time, bound: O.Topo, positions, circulation, vinf):
+    """Free-wake update as a composition of reference functions: every filled node's velocity
+    is the free stream + the bound mesh's induction (python/pyvl/wake_models.py:124-126) +
+    the wake's own get_velocity (:159-175), all from the state at the start of the update;
+    then the reference's update bookkeeping (:114-155).  Point-velocity sums go through
+    O.point_velocity (the tensor contraction without the tensor)."""
+    dt = time - ow.current_time
+    if ow.step_count == 0:
+        ow.wake_positions[:, 0, 0, :] = positions[ow.bordering_nodes[:, 0], :]
+        ow.wake_positions[:, 1, 0, :] = positions[ow.bordering_nodes[:, 1], :]
+        ow.step_count = 1
+    cr = min(ow.step_count, ow.line_rows)
+    old = ow.wake_positions[:, :, :cr, :].copy()
+    flat = np.ascontiguousarray(old.reshape(-1, 3))
+    vel = np.full_like(flat, vinf)
+    vel += O.point_velocity(bound, positions, circulation, flat, ow.vortex_tol)
+    vel += O.point_velocity(ow.wake_mesh, ow.wake_positions.reshape(-1, 3), ow.circulation.reshape(-1), flat,
+                            ow.vortex_tol)
+    ow.wake_positions[:, :, 1:cr + 1, :] = old + vel.reshape(old.shape) * dt
+    ow.current_time = time
+    ow.wake_positions[:, 0, 0, :] = positions[ow.bordering_nodes[:, 0], :]
+    ow.wake_positions[:, 1, 0, :] = positions[ow.bordering_nodes[:, 1], :]
+    if cr != ow.line_rows:
+        ow.wake_positions[:, :, cr + 1:, :] = ow.wake_positions[:, :, cr, :][:, :, None, :]
+    ow.circulation[:, 1:] = ow.circulation[:, :-1]
+    ow.circulation[:, 0] = 0
+    mask = ow.adjacent_surfaces != INVALID
+    ow.circulation[mask[:, 1], 0] += circulation[ow.adjacent_surfaces[mask[:, 1], 1]]
+    ow.circulation[mask[:, 0], 0] -= circulation[ow.adjacent_surfaces[mask[:, 0], 0]]
+    ow.step_count += 1
+
+
+def reference_core_step(it: int, dt: float, bound: O.Topo, p0, frame, vinf, tol, ow: OracleWake):
+    """One iteration of python/pyvl/solver.py:175-239 with the wake, every induction taken
+    from the reference's own C core (oracle/_ref: compute_line_induction and the two line-
+    buffer reductions, as src/meshobject.c calls them) and the reference's numpy steps
+    restated: the reference path itself, for checking at full size.  Returns the
+    circulation."""
+    t = it * dt
+    pos = frame.positions(p0, t)
+    vel = frame.velocities(p0, t)
+    cps = O.ref_surface_center(bound, pos)
+    cpv = O.ref_surface_center(bound, vel)
+    nrm = O.ref_surface_normal(bound, pos)
+    b = np.vecdot(nrm, -(np.full_like(cps, vinf) - cpv), axis=1)
+    a = O.ref_induction_matrix3(bound, pos, cps, nrm, tol)
+    if it > 0:
+        n = ow.shedding_lines.size
+        ind = O.ref_induction_matrix3(ow.wake_mesh, ow.wake_positions.reshape(-1, 3), cps, nrm, tol)
+        ind = ind.reshape(cps.shape[0], n, ow.line_rows)
+        for i_line in range(n):
+            corr = ind[:, i_line, 0]
+            s1, s0 = ow.adjacent_surfaces[i_line, 1], ow.adjacent_surfaces[i_line, 0]
+            if s1 != INVALID:
+                a[:, s1] -= corr
+            if s0 != INVALID:
+                a[:, s0] += corr
+        b += np.vecdot(ind[:, :, 1:].reshape(cps.shape[0], -1), ow.circulation[:, 1:].reshape(-1)[None, :], axis=1)
+    gamma = la.lu_solve(la.lu_factor(a), b)
+    dt_w = t - ow.current_time
+    ow.current_time = t
+    if ow.step_count == 0:
+        ow.wake_positions[:, 0, 0, :] = pos[ow.bordering_nodes[:, 0], :]
+        ow.wake_positions[:, 1, 0, :] = pos[ow.bordering_nodes[:, 1], :]
+        ow.step_count = 1
+    cr = min(ow.step_count, ow.line_rows)
+    for i in reversed(range(cr)):
+        old = ow.wake_positions[:, :, i, :]
+        fl = np.array(old.reshape(-1, 3))
+        v = np.full_like(fl, vinf)
+        v += np.vecdot(O.ref_induction_matrix(bound, pos, fl, tol), gamma[None, :, None], axis=1)
+        ow.wake_positions[:, :, i + 1, :] = old + v.reshape(old.shape) * dt_w
+    ow.wake_positions[:, 0, 0, :] = pos[ow.bordering_nodes[:, 0], :]
+    ow.wake_positions[:, 1, 0, :] = pos[ow.bordering_nodes[:, 1], :]
+    if cr != ow.line_rows:
+        ow.wake_positions[:, :, cr + 1:, :] = ow.wake_positions[:, :, cr, :][:, :, None, :]
+    ow.circulation[:, 1:] = ow.circulation[:, :-1]
+    ow.circulation[:, 0] = 0
+    m = ow.adjacent_surfaces != INVALID
+    ow.circulation[m[:, 1], 0] += gamma[ow.adjacent_surfaces[m[:, 1], 1]]
+    ow.circulation[m[:, 0], 0] -= gamma[ow.adjacent_surfaces[m[:, 0], 0]]
+    ow.step_count += 1
+    return gamma
+
+
+def step(it: int, dt: float, bound: O.Topo, p0, frame, vinf, tol, wake: OracleWake | None, closed=False,
+         free=False):
+    """One iteration of python/pyvl/solver.py:175-239 for a single body (free=True: the
+    wake convects with free_update)."""
     t = it * dt
     pos = frame.positions(p0, t)
     vel = frame.velocities(p0, t)
@@ -117,5 +204,8 @@ def step(it: int, dt: float, bound: O.Topo, p0, frame, vinf, tol, wake: OracleWa
     if closed:
         gamma -= np.mean(gamma)
     if wake is not None:
-        wake.update(t, bound, pos, gamma, vinf)
+        if free:
+            free_update(wake, t, bound, pos, gamma, vinf)
+        else:
+            wake.update(t, bound, pos, gamma, vinf)
     return gamma, a, b

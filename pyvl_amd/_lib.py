@@ -6,6 +6,7 @@ raises.  The signatures mirror ``include/pvl.h``.
 
 from __future__ import annotations
 
+import atexit
 import ctypes as C
 import os
 from pathlib import Path
@@ -35,6 +36,7 @@ SIGNATURES: dict[str, tuple] = {
     "pvl_init": (C.c_int, [C.c_int]),
     "pvl_device_count": (C.c_int, [C.POINTER(C.c_int)]),
     "pvl_synchronize": (C.c_int, []),
+    "pvl_shutdown": (C.c_int, []),
     "pvl_malloc": (C.c_int, [C.POINTER(C.c_void_p), C.c_size_t]),
     "pvl_free": (C.c_int, [_dp]),
     "pvl_memcpy_h2d": (C.c_int, [_dp, _hp, C.c_size_t]),
@@ -144,6 +146,7 @@ SIGNATURES: dict[str, tuple] = {
 }
 
 _LIB: C.CDLL | None = None
+_SHUT = False
 
 
 class PvlError(RuntimeError):
@@ -170,7 +173,23 @@ def load() -> C.CDLL:
         fn.restype = res
         fn.argtypes = args
     _LIB = lib
+    atexit.register(_shutdown)
     return lib
+
+
+def _shutdown() -> None:
+    """Release the library's streams, events, pinned memory and workspaces while the HIP
+    runtime is still up (atexit runs before interpreter and runtime teardown).  Handles
+    still alive afterwards are left to process exit: their finalizers become no-ops."""
+    global _SHUT
+    if _LIB is not None and not _SHUT:
+        _SHUT = True
+        _LIB.pvl_shutdown()
+
+
+def alive() -> bool:
+    """True while handles may still be released through the library."""
+    return _LIB is not None and not _SHUT
 
 
 def check(status: int, what: str = "") -> None:

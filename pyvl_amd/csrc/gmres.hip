@@ -310,6 +310,19 @@ struct GmresWs
 };
 GmresWs g_gw;
 
+} // namespace
+
+void pvl::gmres_release_workspace()
+{
+    for (void *p : {(void *)g_gw.x, (void *)g_gw.result, (void *)g_gw.V, (void *)g_gw.r32, (void *)g_gw.d32,
+                    (void *)g_gw.dinv, (void *)g_gw.r64, (void *)g_gw.norms})
+        (void)hipFree(p);
+    g_gw = GmresWs{};
+}
+
+namespace
+{
+
 int ensure_gmres_ws(int n, int m)
 {
     if (!g_gw.x)
@@ -355,6 +368,7 @@ extern "C" {
 int pvl_gmres_refine(int n, const double *A64_dev, const float *A32_dev, const double *b_dev, double *x_dev, double tol,
                      int m, double inner_tol, int max_outer, int *stats, double *res_out)
 {
+    PVL_API_LOCK;
     if (int st = ensure_init())
         return st;
     if (n <= 0 || m <= 0 || m > kMaxM || max_outer < 0)
@@ -441,8 +455,8 @@ int pvl_gmres_refine(int n, const double *A64_dev, const float *A32_dev, const d
         void *args[] = {&nn, (void *)&Ap, (void *)&dv, (void *)&rh, &dd, &Vp, &mm, &tl, &rp, &Xp, &res};
         {
             ProfScope prof("gmres_cycle");
-            PVL_HIP(hipLaunchCooperativeKernel((const void *)gmres_cycle_kernel, dim3(G), dim3(256), args, 0,
-                                               stream()));
+            if (int st = launch_grid_synced((const void *)gmres_cycle_kernel, G, 256, args, 0, stream()))
+                return st;
         }
         add_scaled_kernel<<<64, 256, 0, stream()>>>(x_dev, g_gw.d32, n, 1.0 / scale);
         PVL_HIP(hipGetLastError());

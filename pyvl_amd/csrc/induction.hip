@@ -597,6 +597,7 @@ extern "C" {
 int pvl_induction_matrix3(const pvl_mesh *m, double tol, const double *pos, unsigned n_cpts, const double *cpts,
                           const double *normals, double *out, int precision)
 {
+    PVL_API_LOCK;
     if (int st = check_common(m, tol, precision))
         return st;
     return launch_surface<true>(m, tol, pos, n_cpts, cpts, normals, out, precision);
@@ -605,6 +606,7 @@ int pvl_induction_matrix3(const pvl_mesh *m, double tol, const double *pos, unsi
 int pvl_induction_matrix(const pvl_mesh *m, double tol, const double *pos, unsigned n_cpts, const double *cpts,
                          double *out, int precision)
 {
+    PVL_API_LOCK;
     if (int st = check_common(m, tol, precision))
         return st;
     return launch_surface<false>(m, tol, pos, n_cpts, cpts, nullptr, out, precision);
@@ -613,6 +615,7 @@ int pvl_induction_matrix(const pvl_mesh *m, double tol, const double *pos, unsig
 int pvl_line_induction_matrix(const pvl_mesh *m, double tol, const double *pos, unsigned n_cpts, const double *cpts,
                               double *out, int precision)
 {
+    PVL_API_LOCK;
     if (int st = check_common(m, tol, precision))
         return st;
     if (n_cpts == 0 || m->n_lines == 0)
@@ -639,6 +642,7 @@ int pvl_line_induction_matrix(const pvl_mesh *m, double tol, const double *pos, 
 int pvl_point_velocity(const pvl_mesh *m, double tol, const double *pos, const double *circ, unsigned n_pts,
                        const double *pts, double *out, int accumulate, int precision)
 {
+    PVL_API_LOCK;
     if (int st = check_common(m, tol, precision))
         return st;
     return pvl::point_velocity_impl(m, tol, pos, circ, n_pts, pts, out, accumulate, precision, m->n_lines);
@@ -646,6 +650,7 @@ int pvl_point_velocity(const pvl_mesh *m, double tol, const double *pos, const d
 
 int pvl_surface_normal(const pvl_mesh *m, const double *pos, double *out)
 {
+    PVL_API_LOCK;
     if (int st = check_common(m, 0.0, PVL_FP64))
         return st;
     if (m->n_surfaces == 0)
@@ -658,6 +663,7 @@ int pvl_surface_normal(const pvl_mesh *m, const double *pos, double *out)
 
 int pvl_surface_center(const pvl_mesh *m, const double *pos, double *out)
 {
+    PVL_API_LOCK;
     if (int st = check_common(m, 0.0, PVL_FP64))
         return st;
     if (m->n_surfaces == 0)
@@ -695,6 +701,7 @@ static int unstage(double *out, const double *d_out, size_t bytes)
 int pvl_induction_matrix3_host(const pvl_mesh *m, double tol, const double *pos, unsigned n_cpts, const double *cpts,
                                const double *normals, double *out, int precision)
 {
+    PVL_API_LOCK;
     if (int st = check_common(m, tol, precision))
         return st;
     double *dp, *dc, *dn;
@@ -712,6 +719,7 @@ int pvl_induction_matrix3_host(const pvl_mesh *m, double tol, const double *pos,
 int pvl_induction_matrix_host(const pvl_mesh *m, double tol, const double *pos, unsigned n_cpts, const double *cpts,
                               double *out, int precision)
 {
+    PVL_API_LOCK;
     if (int st = check_common(m, tol, precision))
         return st;
     double *dp, *dc, *dn;
@@ -729,6 +737,7 @@ int pvl_induction_matrix_host(const pvl_mesh *m, double tol, const double *pos, 
 int pvl_line_induction_matrix_host(const pvl_mesh *m, double tol, const double *pos, unsigned n_cpts,
                                    const double *cpts, double *out, int precision)
 {
+    PVL_API_LOCK;
     if (int st = check_common(m, tol, precision))
         return st;
     double *dp, *dc, *dn;

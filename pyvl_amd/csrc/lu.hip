@@ -1317,6 +1317,16 @@ __global__ void status_to_host_kernel(const int *__restrict__ status, int *__res
         host[threadIdx.x] = status[threadIdx.x];
 }
 
+// A timed-out wait of the single-launch solve (err[5]) is recorded in the pinned host word
+// kAsyncErrorWord, where it stays until the host takes it at its next synchronisation.
+constexpr int kAsyncErrorWord = 8;
+__global__ void record_async_error_kernel(const int *__restrict__ err, int *__restrict__ host)
+{
+    main_stream_priority();
+    if (threadIdx.x == 0 && *err)
+        host[kAsyncErrorWord] = 1;
+}
+
 __global__ void identity_kernel(int *__restrict__ perm, int n)
 {
     main_stream_priority();
@@ -1334,11 +1344,40 @@ int ensure_ws()
         PVL_HIP(hipMemset(g_ws.err, 0, 8 * sizeof(int)));
         void *h = nullptr;
         PVL_HIP(hipHostMalloc(&h, 16 * sizeof(int), hipHostMallocCoherent | hipHostMallocMapped));
+        for (int i = 0; i < 16; ++i)
+            ((int *)h)[i] = 0;
         g_status_host = (volatile int *)h;
         PVL_HIP(hipHostGetDevicePointer((void **)&g_status_host_dev, h, 0));
     }
     return PVL_OK;
 }
+
+} // namespace
+
+int pvl::take_async_error()
+{
+    if (g_status_host && g_status_host[kAsyncErrorWord])
+    {
+        g_status_host[kAsyncErrorWord] = 0;
+        set_error("a single-launch triangular solve timed out waiting for a block (its rows were written as NaN)");
+        return PVL_ERR_HIP;
+    }
+    return PVL_OK;
+}
+
+void pvl::lu_release_workspace()
+{
+    (void)hipFree(g_ws.panelx);
+    (void)hipFree(g_ws.err);
+    if (g_status_host)
+        (void)hipHostFree((void *)g_status_host);
+    g_ws = Workspace{};
+    g_status_host = nullptr;
+    g_status_host_dev = nullptr;
+}
+
+namespace
+{
 
 template <typename T> int panel_rows_max() { return 256; }
 
@@ -1375,7 +1414,7 @@ template <typename T> int factor_impl(int n, T *A, int *piv, int *info_out, bool
             return PVL_OK;
         const int nt = (n2 + 63) / 64, cpx = (nt + kXcds - 1) / kXcds;
         const int mt = (m2 + 63) / 64;
-        ProfScope prof(sizeof(T) == 8 ? "lu_gemm_f64" : "lu_gemm_f32");
+        ProfScope prof(sizeof(T) == 8 ? "lu_gemm_f64" : "lu_gemm_f32", 2.0 * m2 * n2 * K);
         bool vec = false;
         if constexpr (sizeof(T) == 4)
             vec = n % 4 == 0;
@@ -1442,8 +1481,8 @@ template <typename T> int factor_impl(int n, T *A, int *piv, int *info_out, bool
             const size_t lds_now = sizeof(T) * (size_t)R * (NB + 1);
             {
                 ProfScope prof(sizeof(T) == 8 ? "lu_panel_f64" : "lu_panel_f32");
-                PVL_HIP(hipLaunchCooperativeKernel((const void *)panel_kernel<T>, dim3(G), dim3(256), args, lds_now,
-                                                   stream()));
+                if (int st = launch_grid_synced((const void *)panel_kernel<T>, G, 256, args, lds_now, stream()))
+                    return st;
             }
         }
         // info / err readback is deferred to the end (one sync per factorisation)
@@ -1492,6 +1531,8 @@ template <typename T> int factor_impl(int n, T *A, int *piv, int *info_out, bool
     status_to_host_kernel<<<1, 64, 0, stream()>>>(status, g_status_host_dev);
     PVL_HIP(hipGetLastError());
     PVL_HIP(hipStreamSynchronize(stream()));
+    if (int st = take_async_error())
+        return st;
     int st2[4];
     for (int i = 0; i < 4; ++i)
         st2[i] = g_status_host[i];
@@ -1617,6 +1658,7 @@ template <typename T> int solve_impl(pvl_lu *h, T *B, int nrhs)
                     trsv_flow_kernel<T, false, BS, NT><<<nb, NT, 0, stream()>>>(Ap, n, n, work, nq, h->dinv, h->flow,
                                                                                h->epoch, g_ws.err + 5);
             }
+            record_async_error_kernel<<<1, 64, 0, stream()>>>(g_ws.err + 5, g_status_host_dev);
             PVL_HIP(hipGetLastError());
         }
         if (int e = device_copy(B, work, sizeof(T) * (size_t)n * nq))
@@ -1642,6 +1684,7 @@ extern "C" {
 
 int pvl_lu_create(pvl_lu **out, int n, int precision)
 {
+    PVL_API_LOCK;
     *out = nullptr;
     if (int st = ensure_init())
         return st;
@@ -1674,6 +1717,7 @@ int pvl_lu_create(pvl_lu **out, int n, int precision)
 
 int pvl_lu_destroy(pvl_lu *h)
 {
+    PVL_API_LOCK;
     if (!h)
         return PVL_OK;
     (void)hipStreamSynchronize(stream());
@@ -1694,6 +1738,7 @@ int pvl_lu_destroy(pvl_lu *h)
 // enough (refinement reports PVL_ERR_NOTCONV otherwise).
 int pvl_lu_set_pivoting(pvl_lu *h, int pivoting)
 {
+    PVL_API_LOCK;
     if (!h || (pivoting != PVL_PIVOT_NONE && pivoting != PVL_PIVOT_PARTIAL && pivoting != PVL_PIVOT_AUTO))
     {
         set_error("lu_set_pivoting: %s", h ? "unknown mode" : "null LU handle");
@@ -1707,6 +1752,7 @@ int pvl_lu_set_pivoting(pvl_lu *h, int pivoting)
 // handle's precision and factors it.  info_out = first zero pivot (1-based) or 0.
 int pvl_lu_factor(pvl_lu *h, const double *A_dev, int *info_out)
 {
+    PVL_API_LOCK;
     if (int st = ensure_init())
         return st;
     const int n = h->n;
@@ -1754,6 +1800,7 @@ int pvl_lu_factor(pvl_lu *h, const double *A_dev, int *info_out)
 // handle) is overwritten with X.
 int pvl_lu_solve(pvl_lu *h, void *B_dev, int nrhs)
 {
+    PVL_API_LOCK;
     if (int st = ensure_init())
         return st;
     if (nrhs <= 0 || nrhs > kMaxRhs)
@@ -1768,6 +1815,7 @@ int pvl_lu_solve(pvl_lu *h, void *B_dev, int nrhs)
 
 int pvl_lu_pivots(const pvl_lu *h, int *piv_host)
 {
+    PVL_API_LOCK;
     PVL_HIP(hipMemcpyAsync(piv_host, h->piv, sizeof(int) * h->n, hipMemcpyDeviceToHost, stream()));
     PVL_HIP(hipStreamSynchronize(stream()));
     return PVL_OK;
@@ -1775,6 +1823,7 @@ int pvl_lu_pivots(const pvl_lu *h, int *piv_host)
 
 int pvl_lu_factors(const pvl_lu *h, void *lu_dev_out)
 {
+    PVL_API_LOCK;
     const size_t es = h->prec == PVL_FP64 ? 8 : 4;
     PVL_HIP(hipMemcpyAsync(lu_dev_out, h->lu, es * (size_t)h->n * h->n, hipMemcpyDeviceToDevice, stream()));
     return PVL_OK;
@@ -1783,6 +1832,7 @@ int pvl_lu_factors(const pvl_lu *h, void *lu_dev_out)
 // R = B - A X in FP64 (A FP64, n x n; X, B, R n x nrhs).
 int pvl_residual(int n, const double *A_dev, const double *X_dev, const double *B_dev, double *R_dev, int nrhs)
 {
+    PVL_API_LOCK;
     if (int st = ensure_init())
         return st;
     if (nrhs <= 0 || nrhs > kMaxRhs)
@@ -1821,6 +1871,7 @@ int pvl_residual(int n, const double *A_dev, const double *X_dev, const double *
 int pvl_solve_refine(pvl_lu *h, const double *A_dev, const double *B_dev, double *X_dev, int nrhs, double tol,
                      int max_iter, int *iters_out, double *res_out)
 {
+    PVL_API_LOCK;
     if (int st = ensure_init())
         return st;
     if (h->prec != PVL_FP32)
@@ -1864,6 +1915,8 @@ int pvl_solve_refine(pvl_lu *h, const double *A_dev, const double *B_dev, double
         PVL_HIP(hipGetLastError());
         PVL_HIP(hipMemcpyAsync(hn.data(), norms, sizeof(double) * 2 * nrhs, hipMemcpyDeviceToHost, stream()));
         PVL_HIP(hipStreamSynchronize(stream()));
+        if (int st = take_async_error())
+            return st;
         worst = 0.0;
         for (int q = 0; q < nrhs; ++q)
         {
@@ -1902,6 +1955,7 @@ int pvl_solve_refine(pvl_lu *h, const double *A_dev, const double *B_dev, double
 
 int pvl_convert_f64_f32(const double *src, float *dst, size_t count)
 {
+    PVL_API_LOCK;
     if (int st = ensure_init())
         return st;
     if (count == 0)
@@ -1927,6 +1981,7 @@ __global__ void deflate_kernel(double *A, int n, int r0, int r1, double c)
 
 int pvl_deflate(int n, double *A_dev, int r0, int r1, double c)
 {
+    PVL_API_LOCK;
     if (int st = ensure_init())
         return st;
     if (r0 < 0 || r1 > n || r0 >= r1)

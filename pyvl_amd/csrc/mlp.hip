@@ -290,6 +290,7 @@ int pvl_mlp_destroy(pvl_mlp *h);
 
 int pvl_mlp_create(pvl_mlp **out, int d_in, int width, int n_layers)
 {
+    PVL_API_LOCK;
     *out = nullptr;
     if (int st = ensure_init())
         return st;
@@ -341,6 +342,7 @@ int pvl_mlp_create(pvl_mlp **out, int d_in, int width, int n_layers)
 
 int pvl_mlp_destroy(pvl_mlp *h)
 {
+    PVL_API_LOCK;
     if (!h)
         return PVL_OK;
     (void)hipStreamSynchronize(stream());
@@ -363,6 +365,7 @@ int pvl_mlp_destroy(pvl_mlp *h)
 
 int pvl_mlp_num_params(const pvl_mlp *h, long *n)
 {
+    PVL_API_LOCK;
     *n = (long)h->n_params;
     return PVL_OK;
 }
@@ -370,6 +373,7 @@ int pvl_mlp_num_params(const pvl_mlp *h, long *n)
 // Parameters (host FP32, layout: for each layer W (out x in, row-major) then b (out)).
 int pvl_mlp_set_params(pvl_mlp *h, const float *params_host)
 {
+    PVL_API_LOCK;
     if (int st = ensure_init())
         return st;
     PVL_HIP(hipMemcpyAsync(h->d_p, params_host, sizeof(float) * h->n_params, hipMemcpyHostToDevice, stream()));
@@ -384,6 +388,7 @@ int pvl_mlp_set_params(pvl_mlp *h, const float *params_host)
 
 int pvl_mlp_get_params(const pvl_mlp *h, float *params_host)
 {
+    PVL_API_LOCK;
     PVL_HIP(hipMemcpyAsync(params_host, h->d_p, sizeof(float) * h->n_params, hipMemcpyDeviceToHost, stream()));
     PVL_HIP(hipStreamSynchronize(stream()));
     return PVL_OK;
@@ -392,6 +397,7 @@ int pvl_mlp_get_params(const pvl_mlp *h, float *params_host)
 // Inference: Y[b] = net(X[b, :]) for X (batch x d_in, FP32 device), Y FP32 device.
 int pvl_mlp_forward(pvl_mlp *h, const float *X_dev, float *Y_dev, int batch)
 {
+    PVL_API_LOCK;
     if (int st = ensure_init())
         return st;
     if (batch <= 0)
@@ -408,6 +414,7 @@ int pvl_mlp_forward(pvl_mlp *h, const float *X_dev, float *Y_dev, int batch)
 int pvl_mlp_gradient(pvl_mlp *h, const float *X_dev, const float *T_dev, int batch, float *grads_host,
                      float *loss_host)
 {
+    PVL_API_LOCK;
     if (int st = ensure_init())
         return st;
     if (batch <= 0)
@@ -432,6 +439,7 @@ int pvl_mlp_gradient(pvl_mlp *h, const float *X_dev, const float *T_dev, int bat
 int pvl_mlp_train_step(pvl_mlp *h, const float *X_dev, const float *T_dev, int batch, float lr, float beta1,
                        float beta2, float eps, float wd, float *loss_host)
 {
+    PVL_API_LOCK;
     if (int st = ensure_init())
         return st;
     if (batch <= 0)

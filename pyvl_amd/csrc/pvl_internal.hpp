@@ -5,6 +5,7 @@
 
 #include <cstdint>
 #include <cstdio>
+#include <mutex>
 #include <string>
 
 #include "../../include/pvl.h"
@@ -14,6 +15,33 @@ namespace pvl
 
 constexpr uint32_t kInvalidId = 0x7FFFFFFFu; // src/core/common.h:54
 constexpr uint32_t kReversed = 0x80000000u;  // src/core/common.h:55
+
+// The library's scratch slots, LU/GMRES workspaces, streams and profiler tables are shared
+// by every caller in the process.  Each entry point that touches them holds this lock from
+// its first use of that state until it returns, so concurrent calls from several host
+// threads are serialised and their device work is ordered on the library stream exactly
+// as if one thread had issued it.  (The reference releases the GIL around its compute,
+// src/meshobject.c:462-468, so callers may well be concurrent.)  Recursive: entry points
+// call each other.
+std::recursive_mutex &api_mutex();
+#define PVL_API_LOCK std::lock_guard<std::recursive_mutex> _pvl_api_lock(::pvl::api_mutex())
+
+// Errors raised by device work after its entry point returned (a single-launch triangular
+// solve whose bounded wait timed out): kernels record them in pinned host memory; this
+// returns PVL_ERR_HIP with a message (and clears the record) once the stream that ran them
+// has been synchronised, PVL_OK otherwise.
+int take_async_error();
+
+// Launch of a kernel whose workgroups wait for each other (grid barriers), with grid size
+// checked against what the device holds at once.  By default a cooperative launch; with
+// PVL_COOPERATIVE=0 in the environment an ordinary launch of the same grid (every
+// grid-wide wait in the library is bounded and reports an error rather than hanging), for
+// tools that cannot follow cooperative launches.
+int launch_grid_synced(const void *func, unsigned grid, unsigned block, void **args, size_t lds, hipStream_t s);
+
+// Release of the process-lifetime workspaces (pvl_shutdown).
+void lu_release_workspace();
+void gmres_release_workspace();
 
 // Error state (thread local) and the library stream.
 void set_error(const char *fmt, ...);

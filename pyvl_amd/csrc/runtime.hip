@@ -17,6 +17,12 @@ static hipStream_t g_stream = nullptr;
 static int g_device = -1;
 static std::mutex g_mutex;
 
+std::recursive_mutex &api_mutex()
+{
+    static std::recursive_mutex m;
+    return m;
+}
+
 void set_error(const char *fmt, ...)
 {
     va_list ap;
@@ -97,6 +103,34 @@ int side_groups()
         groups = per_cu * cus;
     }
     return groups;
+}
+
+static bool cooperative_launches()
+{
+    static int on = -1;
+    if (on < 0)
+    {
+        const char *e = getenv("PVL_COOPERATIVE");
+        on = !(e && e[0] == '0');
+    }
+    return on != 0;
+}
+
+int launch_grid_synced(const void *func, unsigned grid, unsigned block, void **args, size_t lds, hipStream_t s)
+{
+    int per_cu = 0, cus = 0;
+    PVL_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, func, (int)block, lds));
+    PVL_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, g_device < 0 ? 0 : g_device));
+    if ((long)grid > (long)per_cu * cus)
+    {
+        set_error("grid of %u workgroups exceeds the %d x %d that can be resident at once", grid, per_cu, cus);
+        return PVL_ERR_ARG;
+    }
+    if (cooperative_launches())
+        PVL_HIP(hipLaunchCooperativeKernel(func, dim3(grid), dim3(block), args, (unsigned)lds, s));
+    else
+        PVL_HIP(hipLaunchKernel(func, dim3(grid), dim3(block), args, lds, s));
+    return PVL_OK;
 }
 
 // Second library stream (default priority) for work that runs beside the library stream;
@@ -182,8 +216,10 @@ void *scratch(int slot, size_t bytes)
         return s.first;
     if (s.first)
     {
-        // Make sure no queued kernel still reads the old buffer.
+        // Make sure no queued kernel (on either library stream) still reads the old buffer.
         (void)hipStreamSynchronize(g_stream);
+        if (g_side)
+            (void)hipStreamSynchronize(g_side);
         (void)hipFree(s.first);
         s = {nullptr, 0};
     }
@@ -295,6 +331,7 @@ const char *pvl_last_error(void) { return g_err; }
 
 int pvl_init(int device)
 {
+    PVL_API_LOCK;
     std::lock_guard<std::mutex> lk(g_mutex);
     if (g_device >= 0)
     {
@@ -317,16 +354,56 @@ int pvl_device_count(int *count)
 
 int pvl_synchronize(void)
 {
+    PVL_API_LOCK;
     if (int st = ensure_init())
         return st;
     PVL_HIP(hipStreamSynchronize(g_stream));
     if (g_side)
         PVL_HIP(hipStreamSynchronize(g_side));
+    return take_async_error();
+}
+
+int pvl_shutdown(void)
+{
+    PVL_API_LOCK;
+    std::lock_guard<std::mutex> lk(g_mutex);
+    if (g_device < 0)
+        return PVL_OK;
+    (void)hipSetDevice(g_device);
+    if (g_stream)
+        (void)hipStreamSynchronize(g_stream);
+    if (g_side)
+        (void)hipStreamSynchronize(g_side);
+    resolve_pending();
+    for (hipEvent_t e : g_event_pool)
+        (void)hipEventDestroy(e);
+    g_event_pool.clear();
+    g_prof_tab.clear();
+    if (g_t0)
+        (void)hipEventDestroy(g_t0);
+    if (g_t1)
+        (void)hipEventDestroy(g_t1);
+    g_t0 = g_t1 = nullptr;
+    for (auto &sl : g_scratch)
+    {
+        if (sl.first)
+            (void)hipFree(sl.first);
+        sl = {nullptr, 0};
+    }
+    lu_release_workspace();
+    gmres_release_workspace();
+    if (g_side)
+        (void)hipStreamDestroy(g_side);
+    if (g_stream)
+        (void)hipStreamDestroy(g_stream);
+    g_side = g_stream = nullptr;
+    g_device = -1;
     return PVL_OK;
 }
 
 int pvl_malloc(void **dev_ptr, size_t bytes)
 {
+    PVL_API_LOCK;
     if (int st = ensure_init())
         return st;
     *dev_ptr = nullptr;
@@ -338,6 +415,7 @@ int pvl_malloc(void **dev_ptr, size_t bytes)
 
 int pvl_free(void *dev_ptr)
 {
+    PVL_API_LOCK;
     if (!dev_ptr)
         return PVL_OK;
     if (int st = ensure_init())
@@ -349,6 +427,7 @@ int pvl_free(void *dev_ptr)
 
 int pvl_memcpy_h2d(void *dst, const void *src, size_t bytes)
 {
+    PVL_API_LOCK;
     if (int st = ensure_init())
         return st;
     if (bytes == 0)
@@ -360,17 +439,19 @@ int pvl_memcpy_h2d(void *dst, const void *src, size_t bytes)
 
 int pvl_memcpy_d2h(void *dst, const void *src, size_t bytes)
 {
+    PVL_API_LOCK;
     if (int st = ensure_init())
         return st;
     if (bytes == 0)
         return PVL_OK;
     PVL_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, g_stream));
     PVL_HIP(hipStreamSynchronize(g_stream));
-    return PVL_OK;
+    return take_async_error();
 }
 
 int pvl_memcpy_d2d(void *dst, const void *src, size_t bytes)
 {
+    PVL_API_LOCK;
     if (int st = ensure_init())
         return st;
     return device_copy(dst, src, bytes, g_stream);
@@ -378,6 +459,7 @@ int pvl_memcpy_d2d(void *dst, const void *src, size_t bytes)
 
 int pvl_memset_zero(void *dst, size_t bytes)
 {
+    PVL_API_LOCK;
     if (int st = ensure_init())
         return st;
     if (bytes == 0)
@@ -390,12 +472,14 @@ int pvl_memset_zero(void *dst, size_t bytes)
 
 int pvl_prof_filter(const char *name)
 {
+    PVL_API_LOCK;
     g_prof_only = name ? name : "";
     return PVL_OK;
 }
 
 int pvl_prof_enable(int on)
 {
+    PVL_API_LOCK;
     if (int st = ensure_init())
         return st;
     g_prof = on != 0;
@@ -404,6 +488,7 @@ int pvl_prof_enable(int on)
 
 int pvl_prof_reset(void)
 {
+    PVL_API_LOCK;
     resolve_pending();
     g_prof_tab.clear();
     return PVL_OK;
@@ -411,6 +496,7 @@ int pvl_prof_reset(void)
 
 int pvl_prof_query(const char *name, double *total_ms, long *count)
 {
+    PVL_API_LOCK;
     resolve_pending();
     auto it = g_prof_tab.find(name);
     if (it == g_prof_tab.end())
@@ -426,6 +512,7 @@ int pvl_prof_query(const char *name, double *total_ms, long *count)
 
 int pvl_prof_work(const char *name, double *work)
 {
+    PVL_API_LOCK;
     resolve_pending();
     auto it = g_prof_tab.find(name);
     *work = it == g_prof_tab.end() ? 0.0 : it->second.work;
@@ -434,6 +521,7 @@ int pvl_prof_work(const char *name, double *work)
 
 int pvl_timer_start(void)
 {
+    PVL_API_LOCK;
     if (int st = ensure_init())
         return st;
     if (!g_t0)
@@ -447,6 +535,7 @@ int pvl_timer_start(void)
 
 int pvl_timer_stop(double *ms)
 {
+    PVL_API_LOCK;
     if (!g_t0)
     {
         set_error("pvl_timer_stop without pvl_timer_start");

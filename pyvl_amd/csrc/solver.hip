@@ -50,6 +50,7 @@ extern "C" {
 int pvl_rhs_normal_velocity(unsigned n, const double *normals_dev, const double *cpvel_dev, const double *vinf,
                             double *out_dev)
 {
+    PVL_API_LOCK;
     if (int st = ensure_init())
         return st;
     if (n == 0)
@@ -61,6 +62,7 @@ int pvl_rhs_normal_velocity(unsigned n, const double *normals_dev, const double 
 
 int pvl_remove_mean(double *x_dev, int r0, int r1)
 {
+    PVL_API_LOCK;
     if (int st = ensure_init())
         return st;
     if (r0 < 0 || r1 <= r0)

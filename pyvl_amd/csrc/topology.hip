@@ -195,6 +195,7 @@ int pvl_wake_topology(unsigned n_strips, unsigned line_rows, unsigned *n_lines_o
 int pvl_mesh_create(pvl_mesh **out, unsigned n_points, unsigned n_lines, const uint32_t *lines, unsigned n_surfaces,
                     const uint32_t *surface_offsets, const uint32_t *surface_lines)
 {
+    PVL_API_LOCK;
     *out = nullptr;
     if (int st = ensure_init())
         return st;
@@ -285,6 +286,7 @@ int pvl_mesh_create(pvl_mesh **out, unsigned n_points, unsigned n_lines, const u
 
 int pvl_mesh_destroy(pvl_mesh *m)
 {
+    PVL_API_LOCK;
     if (!m)
         return PVL_OK;
     (void)hipStreamSynchronize(stream());
@@ -299,6 +301,7 @@ int pvl_mesh_destroy(pvl_mesh *m)
 
 int pvl_mesh_counts(const pvl_mesh *m, unsigned *n_points, unsigned *n_lines, unsigned *n_surfaces)
 {
+    PVL_API_LOCK;
     *n_points = m->n_points;
     *n_lines = m->n_lines;
     *n_surfaces = m->n_surfaces;

@@ -240,6 +240,7 @@ int pvl_wake_destroy(pvl_wake *w);
 int pvl_wake_create(pvl_wake **out, unsigned n_shed, unsigned line_rows, const uint32_t *bordering_nodes,
                     const uint32_t *adjacent_surfaces, double vortex_tol, double time)
 {
+    PVL_API_LOCK;
     *out = nullptr;
     if (int st = ensure_init())
         return st;
@@ -321,6 +322,7 @@ int pvl_wake_create(pvl_wake **out, unsigned n_shed, unsigned line_rows, const u
 
 int pvl_wake_destroy(pvl_wake *w)
 {
+    PVL_API_LOCK;
     if (!w)
         return PVL_OK;
     (void)hipStreamSynchronize(stream());
@@ -350,6 +352,7 @@ int pvl_wake_destroy(pvl_wake *w)
 
 int pvl_wake_info(const pvl_wake *w, unsigned *n_te, unsigned *n_lines, unsigned *step_count, double *current_time)
 {
+    PVL_API_LOCK;
     if (n_te)
         *n_te = w->n_te;
     if (n_lines)
@@ -385,6 +388,7 @@ static int drop_pending(pvl_wake *w)
 
 int pvl_wake_begin_update(pvl_wake *w, int mode, int precision)
 {
+    PVL_API_LOCK;
     if (int st = ensure_init())
         return st;
     if (mode != 0 && mode != 1)
@@ -440,6 +444,7 @@ int pvl_wake_begin_update(pvl_wake *w, int mode, int precision)
 int pvl_wake_update(pvl_wake *w, double time, const pvl_mesh *bound, const double *bound_pos_dev,
                     const double *circulation_dev, const double *vinf, int mode, int precision)
 {
+    PVL_API_LOCK;
     if (int st = ensure_init())
         return st;
     if (mode != 0 && mode != 1)
@@ -510,6 +515,7 @@ int pvl_wake_update(pvl_wake *w, double time, const pvl_mesh *bound, const doubl
 int pvl_wake_apply_corrections(pvl_wake *w, unsigned n_cpts, const double *cpts_dev, const double *normals_dev,
                                double *mat_dev, unsigned n_surf, double *rhs_dev, int precision)
 {
+    PVL_API_LOCK;
     if (int st = ensure_init())
         return st;
     if (w->step_count == 0 || n_cpts == 0)
@@ -560,6 +566,7 @@ int pvl_wake_apply_corrections(pvl_wake *w, unsigned n_cpts, const double *cpts_
 int pvl_wake_get_velocity(pvl_wake *w, unsigned n_pts, const double *pts_dev, double *out_dev, int accumulate,
                           int precision)
 {
+    PVL_API_LOCK;
     if (int st = ensure_init())
         return st;
     const unsigned na = active_lines(w);
@@ -577,6 +584,7 @@ int pvl_wake_get_velocity(pvl_wake *w, unsigned n_pts, const double *pts_dev, do
 // State in the reference layout: positions (n_shed, 2, R+1, 3), circulation (n_shed, R).
 int pvl_wake_get_state(pvl_wake *w, double *positions_host, double *circulation_host)
 {
+    PVL_API_LOCK;
     if (int st = ensure_init())
         return st;
     const unsigned R = w->R, n_te = w->n_te, n_shed = w->n_shed;
@@ -604,6 +612,7 @@ int pvl_wake_get_state(pvl_wake *w, double *positions_host, double *circulation_
 int pvl_wake_set_state(pvl_wake *w, const double *positions_host, const double *circulation_host, unsigned step_count,
                        double current_time)
 {
+    PVL_API_LOCK;
     if (int st = ensure_init())
         return st;
     const unsigned R = w->R, n_te = w->n_te, n_shed = w->n_shed;

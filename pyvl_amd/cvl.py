@@ -418,7 +418,7 @@ class Mesh:
 
     def __del__(self):
         try:
-            if self._dev is not None and _lib._LIB is not None:
+            if self._dev is not None and _lib.alive():
                 _lib._LIB.pvl_mesh_destroy(self._dev)
                 self._dev = None
         except Exception:

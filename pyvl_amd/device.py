@@ -95,7 +95,7 @@ class DeviceArray:
 
     def __del__(self):
         try:
-            if self._owner is None and self.ptr and _lib._LIB is not None:
+            if self._owner is None and self.ptr and _lib.alive():
                 _lib._LIB.pvl_free(C.c_void_p(self.ptr))
                 self.ptr = 0
         except Exception:

@@ -106,7 +106,7 @@ class LU:
 
     def __del__(self):
         try:
-            if getattr(self, "_h", None) is not None and _lib._LIB is not None:
+            if getattr(self, "_h", None) is not None and _lib.alive():
                 _lib._LIB.pvl_lu_destroy(self._h)
                 self._h = None
         except Exception:

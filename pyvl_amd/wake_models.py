@@ -176,7 +176,7 @@ class WakeModelLineExplicitUnsteady:
 
     def __del__(self):
         try:
-            if getattr(self, "_h", None) is not None and _lib._LIB is not None:
+            if getattr(self, "_h", None) is not None and _lib.alive():
                 _lib._LIB.pvl_wake_destroy(self._h)
                 self._h = None
         except Exception:

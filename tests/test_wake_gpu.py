@@ -102,44 +102,11 @@ def test_update_matches_oracle_teacher_forced(mode):
         if mode == "reference":
             ow.update(0.1 * k, t, ppos, gamma, vinf)
         else:
-            _oracle_free_update(ow, 0.1 * k, t, ppos, gamma, vinf)
+            WO.free_update(ow, 0.1 * k, t, ppos, gamma, vinf)
         wp, wc = gw.state()
         assert np.array_equal(wc, ow.circulation), k
         assert _rel(wp, ow.wake_positions) < 1e-12, k
         assert gw.step_count == ow.step_count
-
-
-def _oracle_free_update(ow, time, bound, positions, circulation, vinf):
-    """Free wake as a composition of reference functions: every node's velocity (free
-    stream + bound induction + the wake's get_velocity) from the state at the start of the
-    update, then the reference's update bookkeeping."""
-    dt = time - ow.current_time
-    if ow.step_count == 0:
-        ow.wake_positions[:, 0, 0, :] = positions[ow.bordering_nodes[:, 0], :]
-        ow.wake_positions[:, 1, 0, :] = positions[ow.bordering_nodes[:, 1], :]
-        ow.step_count = 1
-    cr = min(ow.step_count, ow.line_rows)
-    old = ow.wake_positions[:, :, :cr, :].copy()
-    flat = np.ascontiguousarray(old.reshape(-1, 3))
-    vel = np.full_like(flat, vinf)
-    vel += np.vecdot(O.induction_matrix(bound, positions, flat, ow.vortex_tol), circulation[None, :, None], axis=1)
-    vel += ow.get_velocity(flat)
-    ow.wake_positions[:, :, 1:cr + 1, :] = old + vel.reshape(old.shape) * dt
-    # remaining bookkeeping exactly as the reference (no further convection)
-    saved = ow.step_count
-    ow.step_count = 0  # suppress the convection loop in a plain update
-    ow_rows = ow.line_rows
-    ow.current_time = time
-    ow.wake_positions[:, 0, 0, :] = positions[ow.bordering_nodes[:, 0], :]
-    ow.wake_positions[:, 1, 0, :] = positions[ow.bordering_nodes[:, 1], :]
-    if cr != ow_rows:
-        ow.wake_positions[:, :, cr + 1:, :] = ow.wake_positions[:, :, cr, :][:, :, None, :]
-    ow.circulation[:, 1:] = ow.circulation[:, :-1]
-    ow.circulation[:, 0] = 0
-    mask = ow.adjacent_surfaces != WO.INVALID
-    ow.circulation[mask[:, 1], 0] += circulation[ow.adjacent_surfaces[mask[:, 1], 1]]
-    ow.circulation[mask[:, 0], 0] -= circulation[ow.adjacent_surfaces[mask[:, 0], 0]]
-    ow.step_count = saved + 1
 
 
 def test_corrections_and_velocity_match_oracle():
