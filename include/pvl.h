@@ -156,6 +156,17 @@ int pvl_point_velocity(const pvl_mesh *mesh, double tol, const double *positions
  * src/core/mesh.c:48-68 and :24-46). */
 int pvl_surface_normal(const pvl_mesh *mesh, const double *positions_dev, double *out_dev);
 int pvl_surface_center(const pvl_mesh *mesh, const double *positions_dev, double *out_dev);
+/* Mesh.line_forces (src/meshobject.c:1288-1396), the force on every primal line from its
+ * reduced line circulation: for line (p1, p2) whose dual line is (s+, s-),
+ * c = circulation[s+] - circulation[s-] (INVALID_ID ends skipped) and
+ * out[l] = (r_p2 - r_p1) x ((v_p1 + v_p2) * (0.5 c)).  dual_lines_dev holds the dual mesh's
+ * lines (2 uint32 per primal line, surface indices or INVALID_ID; an index at or beyond the
+ * primal's surface count counts as absent), the dual's counts are checked against the
+ * primal's transposed (points <-> surfaces) as the reference does.  circulation_dev[primal
+ * n_surfaces], positions_dev / velocity_dev [primal n_points][3], out_dev [n_lines][3]. */
+int pvl_line_forces(const pvl_mesh *primal, const uint32_t *dual_lines_dev, unsigned dual_n_points,
+                    unsigned dual_n_lines, unsigned dual_n_surfaces, const double *circulation_dev,
+                    const double *positions_dev, const double *velocity_dev, double *out_dev);
 
 /* Host-buffer convenience forms of the three tensor entry points: stage through device
  * memory and return when the result is in out_host.  These are what a C caller such as
@@ -237,6 +248,12 @@ int pvl_wake_info(const pvl_wake *w, unsigned *n_te, unsigned *n_lines, unsigned
  * i.e. get_velocity of :159-175). */
 int pvl_wake_update(pvl_wake *w, double time, const pvl_mesh *bound, const double *bound_pos_dev,
                     const double *circulation_dev, const double *vinf, int mode, int precision);
+/* The same update with any free stream the reference supports (flow.get_velocity(time, nodes)
+ * at :124): flow is the 7-double descriptor {kind, a[3], b[3]}: kind 0 FlowConditionsUniform
+ * (velocity a), kind 1 FlowConditionsRotating (v = (p - a) x b, python/pyvl/flow_conditions.py
+ * :332-446), evaluated at each node's position before the step. */
+int pvl_wake_update_flow(pvl_wake *w, double time, const pvl_mesh *bound, const double *bound_pos_dev,
+                         const double *circulation_dev, const double *flow, int mode, int precision);
 /* Scheduling only, no reference counterpart: starts the wake's own induction of the next
  * pvl_wake_update (mode 1; it needs only the wake's current state) on a second library
  * stream, so that it runs beside the bound-circulation solve issued in between.  That
@@ -294,8 +311,32 @@ int pvl_convert_f64_f32(const double *src_dev, float *dst_dev, size_t count);
 /* out_i = n_i . -(vinf - cpvel_i); cpvel_dev may be NULL (fixed body). */
 int pvl_rhs_normal_velocity(unsigned n, const double *normals_dev, const double *cpvel_dev, const double *vinf,
                             double *out_dev);
+/* out_i = n_i . -(V(cp_i) - cpvel_i) for the free-stream descriptor of pvl_wake_update_flow
+ * (cpts_dev may be NULL for a uniform stream). */
+int pvl_rhs_flow(unsigned n, const double *normals_dev, const double *cpvel_dev, const double *cpts_dev,
+                 const double *flow, double *out_dev);
 /* x[r0:r1] -= mean(x[r0:r1]) (closed-body circulation gauge). */
 int pvl_remove_mean(double *x_dev, int r0, int r1);
+
+/* ------------------------------------------------------------------------------------
+ * Rigid-frame kinematics (python/pyvl/solver.py:180-193; ReferenceFrame transforms
+ * src/referenceframeobject.c:367-450, add_velocity :939-955 and its subclasses'
+ * python/pyvl/reference_frames.py:122-147, :298-322).
+ * ---------------------------------------------------------------------------------- */
+enum pvl_frame_kind
+{
+    PVL_FRAME_NONE = 0,        /* ReferenceFrame: adds no velocity */
+    PVL_FRAME_TRANSLATING = 1, /* v += c */
+    PVL_FRAME_ROTOR = 2,       /* v += p x c (c = angular velocity) */
+    PVL_FRAME_AFFINE = 3,      /* v += W p + c (oscillating frames) */
+};
+/* The time step's chain walk for every point of one body, child frame first: per frame
+ * p = R^T (p - d); v += term(p); v = R^T v, starting from p = p0, v = 0.  terms holds 24
+ * doubles per frame, already evaluated at the step's time: R^T row-major [9], d [3], W
+ * row-major [9], c [3]; kinds one pvl_frame_kind per frame; at most 8 frames.  vel_dev
+ * may be NULL (positions only). */
+int pvl_frame_kinematics(unsigned n_points, const double *p0_dev, unsigned n_frames, const double *terms,
+                         const int *kinds, double *pos_dev, double *vel_dev);
 
 #ifdef __cplusplus
 }

@@ -1,9 +1,17 @@
 """ORACLE -- test infrastructure only (the checker, never the product).
 
 Python entry to the C restatement in ``pvl_oracle.c`` (``liborc.so``) and, where it was
-built, to the reference's own C core compiled unmodified (``_ref/libpyvl_core_ref.so``).
-Only ``tests/``, ``__graft_entry__.smoke()`` and ``bench.py``'s cpu_baseline leg import this
-module.  Parity status of each function is listed in ``oracle/README.md``.
+built, to the reference's own code compiled unmodified: the C core
+(``_ref/libpyvl_core_ref.so``) and the whole C extension module (``_ref/cvl*.so``).  Only
+``tests/``, ``__graft_entry__.smoke()`` and ``bench.py``'s cpu_baseline leg import this
+module.
+
+What pins each function: line_induction, induction_matrix3, induction_matrix,
+surface_center, surface_normal, mesh_from_elements and mesh_dual are bit-equal to the
+reference's own C (tests/test_oracle.py) and checked against closed forms and the reference
+tests' fixtures; point_velocity against induction_matrix contracted with the circulation;
+line_forces bit-equal to the reference's Mesh.line_forces (tests/test_postprocess_oracle.py);
+lu_factor_small against scipy's LAPACK.
 """
 
 from __future__ import annotations
@@ -47,6 +55,7 @@ def load() -> C.CDLL:
         _ORC.orc_mesh_dual.argtypes = [u, u, v, u, v, v, v, v, v]
         _ORC.orc_lu_factor.argtypes = [u, v, v]
         _ORC.orc_lu_factor.restype = C.c_int
+        _ORC.orc_line_forces.argtypes = [u, v, v, v, v, v, v]
     return _ORC
 
 
@@ -56,6 +65,28 @@ def load_ref() -> C.CDLL | None:
     if _REF is None and REF_PATH.exists():
         _REF = C.CDLL(os.fspath(REF_PATH))
     return _REF
+
+
+_REF_CVL = None
+
+
+def load_ref_cvl():
+    """The reference's whole C extension module ``cvl`` (src/*.c + src/core/*.c compiled
+    unmodified by oracle/Makefile into _ref/), imported from its file, or None if it was not
+    built.  Its Mesh and ReferenceFrame are the reference's own C types."""
+    global _REF_CVL
+    if _REF_CVL is None:
+        import importlib.util
+        import sysconfig
+
+        path = HERE / "_ref" / ("cvl" + sysconfig.get_config_var("EXT_SUFFIX"))
+        if not path.exists():
+            return None
+        spec = importlib.util.spec_from_file_location("cvl", path)
+        mod = importlib.util.module_from_spec(spec)
+        spec.loader.exec_module(mod)
+        _REF_CVL = mod
+    return _REF_CVL
 
 
 # ---------------------------------------------------------------------------------------
@@ -97,6 +128,17 @@ def mesh_dual(t: Topo):
     load().orc_mesh_dual(t.n_points, t.n_lines, _p(t.lines), t.n_surfaces, _p(t.offsets), _p(t.entries), _p(dl),
                          _p(do), _p(de))
     return dl[: t.n_lines], do, de[: int(do[-1])]
+
+
+def line_forces(primal: Topo, dual_lines, circ, pos, vel) -> np.ndarray:
+    """Mesh.line_forces (src/meshobject.c:1350-1393) restated: (n_lines, 3)."""
+    dl = np.ascontiguousarray(dual_lines, np.uint32)
+    c = np.ascontiguousarray(circ, np.float64)
+    p = np.ascontiguousarray(pos, np.float64)
+    v = np.ascontiguousarray(vel, np.float64)
+    out = np.zeros((primal.n_lines, 3), np.float64)
+    load().orc_line_forces(primal.n_lines, _p(primal.lines), _p(dl), _p(c), _p(p), _p(v), _p(out))
+    return out
 
 
 def line_induction(t: Topo, pos, cpts, tol) -> np.ndarray:

@@ -387,3 +387,31 @@ int orc_lu_factor(unsigned n, double *a, int *piv)
     }
     return 0;
 }
+
+/* Mesh.line_forces (src/meshobject.c:1350-1393): for primal line l = (p1, p2) with dual line
+ * (s+, s-), the line circulation c = (s+ valid ? circ[s+] : 0) - (s- valid ? circ[s-] : 0)
+ * and f_l = (r_p2 - r_p1) x ((v_p1 + v_p2) * (0.5 c)), with the reference's real3_add,
+ * real3_mul1 and real3_cross (src/core/common.h:98-136) in their operation order. */
+void orc_line_forces(unsigned n_lines, const uint32_t *primal_lines, const uint32_t *dual_lines, const double *circ,
+                     const double *pos, const double *vel, double *f)
+{
+    for (unsigned il = 0; il < n_lines; ++il)
+    {
+        const uint32_t p1 = primal_lines[2 * il] & ORC_MASK, p2 = primal_lines[2 * il + 1] & ORC_MASK;
+        const uint32_t d1 = dual_lines[2 * il] & ORC_MASK, d2 = dual_lines[2 * il + 1] & ORC_MASK;
+        double dr[3], av[3];
+        for (int c = 0; c < 3; ++c)
+            dr[c] = pos[3 * p2 + c] - pos[3 * p1 + c];
+        double lc = 0;
+        if (d1 != ORC_INVALID)
+            lc += circ[d1];
+        if (d2 != ORC_INVALID)
+            lc -= circ[d2];
+        const double k = 0.5 * lc;
+        for (int c = 0; c < 3; ++c)
+            av[c] = (vel[3 * p1 + c] + vel[3 * p2 + c]) * k;
+        f[3 * il + 0] = dr[1] * av[2] - dr[2] * av[1];
+        f[3 * il + 1] = dr[2] * av[0] - dr[0] * av[2];
+        f[3 * il + 2] = dr[0] * av[1] - dr[1] * av[0];
+    }
+}

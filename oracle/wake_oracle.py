@@ -18,9 +18,30 @@ from __future__ import annotations
 import numpy as np
 import scipy.linalg as la
 
+from . import frames_oracle as FO
 from . import oracle as O
 
 INVALID = 0x7FFFFFFF
+
+
+def free_stream(vinf, t, pts):
+    """The free stream at points: ``vinf`` is a uniform (3,) velocity or a field
+    ``vinf(t, pts)`` (flow.get_velocity, python/pyvl/wake_models.py:124)."""
+    if callable(vinf):
+        return np.asarray(vinf(t, pts), np.float64)
+    return np.full_like(pts, vinf)
+
+
+def rotating_flow(center, omega):
+    """FlowConditionsRotating.get_velocity (python/pyvl/flow_conditions.py:412-446):
+    v = (p - center) x omega."""
+    c = np.asarray(center, np.float64)[None, :]
+    w = np.asarray(omega, np.float64)
+
+    def field(t, pts):
+        return np.cross(np.asarray(pts, np.float64) - c, w)
+
+    return field
 
 
 def wake_ring_topology(n_lines: int, line_rows: int) -> O.Topo:
@@ -62,7 +83,7 @@ class OracleWake:
         for i in reversed(range(complete_rows)):
             old_pos = self.wake_positions[:, :, i, :]
             flat = np.array(old_pos.reshape((-1, 3)), np.float64)
-            vel = np.full_like(flat, vinf)
+            vel = free_stream(vinf, time, flat)
             ind_mat = O.induction_matrix(bound, positions, flat, self.vortex_tol)
             vel += np.vecdot(ind_mat, circulation[None, :, None], axis=1)
             new_pos = old_pos + vel.reshape(old_pos.shape) * dt
@@ -114,7 +135,7 @@ def free_update(ow: OracleWake, time, bound: O.Topo, positions, circulation, vin
     cr = min(ow.step_count, ow.line_rows)
     old = ow.wake_positions[:, :, :cr, :].copy()
     flat = np.ascontiguousarray(old.reshape(-1, 3))
-    vel = np.full_like(flat, vinf)
+    vel = free_stream(vinf, time, flat)
     vel += O.point_velocity(bound, positions, circulation, flat, ow.vortex_tol)
     vel += O.point_velocity(ow.wake_mesh, ow.wake_positions.reshape(-1, 3), ow.circulation.reshape(-1), flat,
                             ow.vortex_tol)
@@ -139,12 +160,11 @@ def reference_core_step(it: int, dt: float, bound: O.Topo, p0, frame, vinf, tol,
     restated: the reference path itself, for checking at full size.  Returns the
     circulation."""
     t = it * dt
-    pos = frame.positions(p0, t)
-    vel = frame.velocities(p0, t)
+    pos, vel = FO.kinematics(frame, p0, t)
     cps = O.ref_surface_center(bound, pos)
     cpv = O.ref_surface_center(bound, vel)
     nrm = O.ref_surface_normal(bound, pos)
-    b = np.vecdot(nrm, -(np.full_like(cps, vinf) - cpv), axis=1)
+    b = np.vecdot(nrm, -(free_stream(vinf, t, cps) - cpv), axis=1)
     a = O.ref_induction_matrix3(bound, pos, cps, nrm, tol)
     if it > 0:
         n = ow.shedding_lines.size
@@ -169,7 +189,7 @@ def reference_core_step(it: int, dt: float, bound: O.Topo, p0, frame, vinf, tol,
     for i in reversed(range(cr)):
         old = ow.wake_positions[:, :, i, :]
         fl = np.array(old.reshape(-1, 3))
-        v = np.full_like(fl, vinf)
+        v = free_stream(vinf, t, fl)
         v += np.vecdot(O.ref_induction_matrix(bound, pos, fl, tol), gamma[None, :, None], axis=1)
         ow.wake_positions[:, :, i + 1, :] = old + v.reshape(old.shape) * dt_w
     ow.wake_positions[:, 0, 0, :] = pos[ow.bordering_nodes[:, 0], :]
@@ -190,12 +210,11 @@ def step(it: int, dt: float, bound: O.Topo, p0, frame, vinf, tol, wake: OracleWa
     """One iteration of python/pyvl/solver.py:175-239 for a single body (free=True: the
     wake convects with free_update)."""
     t = it * dt
-    pos = frame.positions(p0, t)
-    vel = frame.velocities(p0, t)
+    pos, vel = FO.kinematics(frame, p0, t)
     cps = O.surface_center(bound, pos)
     cpv = O.surface_center(bound, vel)
     nrm = O.surface_normal(bound, pos)
-    ev = np.full_like(cps, vinf) - cpv
+    ev = free_stream(vinf, t, cps) - cpv
     b = np.vecdot(nrm, -ev, axis=1)
     a = O.induction_matrix3(bound, pos, cps, nrm, tol)
     if wake is not None and it > 0:

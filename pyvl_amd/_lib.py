@@ -78,6 +78,7 @@ SIGNATURES: dict[str, tuple] = {
         C.c_int,
         [C.c_void_p, C.c_double, _dp, _dp, C.c_uint, _dp, _dp, C.c_int, C.c_int],
     ),
+    "pvl_line_forces": (C.c_int, [C.c_void_p, _dp, C.c_uint, C.c_uint, C.c_uint, _dp, _dp, _dp, _dp]),
     "pvl_surface_normal": (C.c_int, [C.c_void_p, _dp, _dp]),
     "pvl_surface_center": (C.c_int, [C.c_void_p, _dp, _dp]),
     "pvl_induction_matrix3_host": (
@@ -115,6 +116,10 @@ SIGNATURES: dict[str, tuple] = {
         C.c_int,
         [C.c_void_p, C.c_double, C.c_void_p, _dp, _dp, C.POINTER(C.c_double), C.c_int, C.c_int],
     ),
+    "pvl_wake_update_flow": (
+        C.c_int,
+        [C.c_void_p, C.c_double, C.c_void_p, _dp, _dp, C.POINTER(C.c_double), C.c_int, C.c_int],
+    ),
     "pvl_wake_begin_update": (C.c_int, [C.c_void_p, C.c_int, C.c_int]),
     "pvl_wake_apply_corrections": (
         C.c_int,
@@ -142,7 +147,12 @@ SIGNATURES: dict[str, tuple] = {
     ),
     "pvl_convert_f64_f32": (C.c_int, [_dp, _dp, C.c_size_t]),
     "pvl_rhs_normal_velocity": (C.c_int, [C.c_uint, _dp, _dp, C.POINTER(C.c_double), _dp]),
+    "pvl_rhs_flow": (C.c_int, [C.c_uint, _dp, _dp, _dp, C.POINTER(C.c_double), _dp]),
     "pvl_remove_mean": (C.c_int, [_dp, C.c_int, C.c_int]),
+    "pvl_frame_kinematics": (
+        C.c_int,
+        [C.c_uint, _dp, C.c_uint, C.POINTER(C.c_double), C.POINTER(C.c_int), _dp, _dp],
+    ),
 }
 
 _LIB: C.CDLL | None = None

@@ -32,9 +32,9 @@ constexpr int kCptTile = 64; // control points per workgroup (one per lane)
 // Per-line prologue (src/core/flow_solver.c:95-103): end points, |d| and d/|d| with three
 // true divisions.
 // A line shorter than tol is cut off for every point (K = 0 in every form); its direction
-// is stored as 0 instead of the 0/0 of a zero-length line, so that the fused accumulation
-// (0 * direction) stays 0.  With tol <= 0 nothing is cut by length and the reference's
-// values, NaN included, are kept.
+// and length are stored as 0 (instead of the 0/0 of a zero-length line), so that its angle
+// and the fused accumulation (0 * direction) are 0.  With tol <= 0 nothing is cut by length
+// and the reference's values, NaN included, are kept.
 __global__ void __launch_bounds__(256) line_geom_kernel(unsigned n_lines, const uint32_t *__restrict__ lines,
                                                        const double *__restrict__ pos, double tol,
                                                        LineGeom *__restrict__ out)
@@ -53,7 +53,7 @@ __global__ void __launch_bounds__(256) line_geom_kernel(unsigned n_lines, const 
     g.ey = dy / g.len;
     g.ez = dz / g.len;
     if (g.len < tol)
-        g.ex = g.ey = g.ez = 0.0;
+        g.ex = g.ey = g.ez = g.len = 0.0;
     out[i] = g;
 }
 
@@ -486,6 +486,153 @@ int launch_surface(const pvl_mesh *m, double tol, const double *pos, unsigned n_
     return PVL_OK;
 }
 
+// ---------------------------------------------------------------------------------------
+// Segment chains (the wake's self-induction).  Prologue: each segment's end node, unit
+// direction in the chain's direction (three true divisions, the reference's line prologue;
+// direction 0 below tol as line_geom_kernel) and circulation, signed for the direction.
+__global__ void __launch_bounds__(256) chain_segment_kernel(unsigned n_seg, const uint32_t *__restrict__ seg_ab,
+                                                            const uint32_t *__restrict__ seg_line,
+                                                            const double *__restrict__ nodes,
+                                                            const double *__restrict__ gamma, unsigned n_active,
+                                                            double tol, SegRec *__restrict__ out)
+{
+    const unsigned j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n_seg)
+        return;
+    const uint32_t a = seg_ab[2 * j], b = seg_ab[2 * j + 1], ln = seg_line[j];
+    const double ax = nodes[3 * a + 0], ay = nodes[3 * a + 1], az = nodes[3 * a + 2];
+    SegRec r;
+    r.nx = nodes[3 * b + 0], r.ny = nodes[3 * b + 1], r.nz = nodes[3 * b + 2];
+    const double dx = r.nx - ax, dy = r.ny - ay, dz = r.nz - az;
+    r.len = sqrt(dx * dx + dy * dy + dz * dz);
+    r.ex = dx / r.len;
+    r.ey = dy / r.len;
+    r.ez = dz / r.len;
+    if (r.len < tol)
+        r.ex = r.ey = r.ez = r.len = 0.0;
+    const uint32_t l = ln & kInvalidId;
+    const double g = l < n_active ? gamma[l] : 0.0;
+    r.gamma = (ln & kReversed) ? -g : g;
+    out[j] = r;
+}
+
+// PPL points per lane; the workgroup's four waves take the pieces of the item's chunk in
+// turn; per-wave sums are combined in LDS in wave order and written per chunk (no atomics).
+// Along a piece, the end node's dr = p - r and |dr|^2 become the next segment's start terms
+// (per pair 8 of the ~96 instructions of line_k_acc are saved).  The segment loop is unrolled
+// by two so that the carried terms are renamed, not copied; 76 VGPRs (capping at 64 spills):
+// four workgroups per CU on the side stream leave each SIMD 208 VGPRs for the library
+// stream's kernels.
+template <int PREC, int PPL>
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6)))
+    chain_velocity_kernel(unsigned n_pts, const double *__restrict__ pts, const double *__restrict__ nodes,
+                          const ChainPiece *__restrict__ pieces, unsigned n_pieces, unsigned cr,
+                          const SegRec *__restrict__ segs, unsigned pieces_per_chunk, double *__restrict__ partial,
+                          double tol, unsigned pblocks, unsigned n_items)
+{
+    __shared__ double red[4][3][kWave * PPL];
+    const unsigned w = wave_id(), lane = lane_id();
+    const double tol2 = tol * tol;
+    for (unsigned item = blockIdx.x; item < n_items; item += gridDim.x)
+    {
+        const unsigned chunk = item / pblocks;
+        const unsigned p0 = (item - chunk * pblocks) * (kWave * PPL) + lane;
+        double px[PPL], py[PPL], pz[PPL], vx[PPL], vy[PPL], vz[PPL];
+#pragma unroll
+        for (int i = 0; i < PPL; ++i)
+        {
+            const unsigned p = p0 + i * kWave;
+            const unsigned pc = p < n_pts ? p : n_pts - 1;
+            px[i] = pts[3 * pc + 0], py[i] = pts[3 * pc + 1], pz[i] = pts[3 * pc + 2];
+            vx[i] = 0.0, vy[i] = 0.0, vz[i] = 0.0;
+        }
+        const unsigned k0 = chunk * pieces_per_chunk;
+        const unsigned k1 = (k0 + pieces_per_chunk) < n_pieces ? (k0 + pieces_per_chunk) : n_pieces;
+        for (unsigned k = k0 + w; k < k1; k += 4)
+        {
+            const ChainPiece pc = pieces[k];
+            unsigned n = pc.n_seg;
+            if (pc.row1)
+            {
+                const unsigned avail = cr - (pc.row1 - 1);
+                n = n < avail ? n : avail;
+            }
+            const double sx = nodes[3 * pc.start_node + 0], sy = nodes[3 * pc.start_node + 1],
+                         sz = nodes[3 * pc.start_node + 2];
+            double ax[PPL], ay[PPL], az[PPL], aq[PPL];
+#pragma unroll
+            for (int i = 0; i < PPL; ++i)
+            {
+                ax[i] = px[i] - sx, ay[i] = py[i] - sy, az[i] = pz[i] - sz;
+                aq[i] = ax[i] * ax[i] + ay[i] * ay[i] + az[i] * az[i];
+            }
+            const SegRec *sg = segs + pc.first_seg;
+#pragma unroll 2
+            for (unsigned j = 0; j < n; ++j)
+            {
+                const SegRec r = sg[j];
+#pragma unroll
+                for (int i = 0; i < PPL; ++i)
+                {
+                    const double bx = px[i] - r.nx, by = py[i] - r.ny, bz = pz[i] - r.nz;
+                    const double bq = bx * bx + by * by + bz * bz;
+                    bool cut;
+                    const double m =
+                        pair_m<PREC>(r.ex, r.ey, r.ez, r.len, ax[i], ay[i], az[i], aq[i], bx, by, bz, bq, tol, tol2, cut);
+                    if constexpr (PREC == PVL_FP64_REF_FORM)
+                    {
+                        const double kx = cut ? 0.0 : (ay[i] * r.ez - az[i] * r.ey) * m;
+                        const double ky = cut ? 0.0 : (az[i] * r.ex - ax[i] * r.ez) * m;
+                        const double kz = cut ? 0.0 : (ax[i] * r.ey - ay[i] * r.ex) * m;
+                        vx[i] = vx[i] + r.gamma * kx;
+                        vy[i] = vy[i] + r.gamma * ky;
+                        vz[i] = vz[i] + r.gamma * kz;
+                    }
+                    else
+                    {
+                        const double wm = cut ? 0.0 : r.gamma * m;
+                        vx[i] = fma(wm, fma(ay[i], r.ez, -(az[i] * r.ey)), vx[i]);
+                        vy[i] = fma(wm, fma(az[i], r.ex, -(ax[i] * r.ez)), vy[i]);
+                        vz[i] = fma(wm, fma(ax[i], r.ey, -(ay[i] * r.ex)), vz[i]);
+                    }
+                    ax[i] = bx, ay[i] = by, az[i] = bz, aq[i] = bq;
+                }
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < PPL; ++i)
+        {
+            red[w][0][i * kWave + lane] = vx[i];
+            red[w][1][i * kWave + lane] = vy[i];
+            red[w][2][i * kWave + lane] = vz[i];
+        }
+        __syncthreads();
+        if (w == 0)
+        {
+#pragma unroll
+            for (int i = 0; i < PPL; ++i)
+            {
+                const unsigned p = p0 + i * kWave;
+                if (p >= n_pts)
+                    continue;
+                const unsigned q = i * kWave + lane;
+                double sx2 = red[0][0][q], sy2 = red[0][1][q], sz2 = red[0][2][q];
+                for (int u = 1; u < 4; ++u)
+                {
+                    sx2 = sx2 + red[u][0][q];
+                    sy2 = sy2 + red[u][1][q];
+                    sz2 = sz2 + red[u][2][q];
+                }
+                double *dst = partial + ((size_t)chunk * n_pts + p) * 3;
+                dst[0] = sx2;
+                dst[1] = sy2;
+                dst[2] = sz2;
+            }
+        }
+        __syncthreads(); // red is rewritten by the next item
+    }
+}
+
 } // namespace
 
 namespace pvl
@@ -550,6 +697,63 @@ int point_velocity_partials(const pvl_mesh *m, double tol, const double *pos, co
         else
             point_velocity_kernel<decltype(P)::value, 0>
                 <<<grid, kBlock, 0, s>>>(n_pts, pts, n_active, g, gamma, lpc, partial, tol, pblocks, n_items);
+    });
+    PVL_HIP(hipGetLastError());
+    return PVL_OK;
+}
+
+int line_circulations(const pvl_mesh *m, unsigned n_active, const double *circ, double *gamma, hipStream_t s)
+{
+    if (n_active == 0)
+        return PVL_OK;
+    line_circulation_kernel<<<(n_active + 255) / 256, 256, 0, s>>>(n_active, m->d_line_offsets, m->d_line_surfs,
+                                                                     circ, gamma);
+    PVL_HIP(hipGetLastError());
+    return PVL_OK;
+}
+
+int chain_segments(unsigned n_seg, const uint32_t *seg_ab, const uint32_t *seg_line, const double *nodes,
+                   const double *gamma, unsigned n_active_lines, double tol, SegRec *out, hipStream_t s)
+{
+    if (n_seg == 0)
+        return PVL_OK;
+    chain_segment_kernel<<<(n_seg + 255) / 256, 256, 0, s>>>(n_seg, seg_ab, seg_line, nodes, gamma, n_active_lines,
+                                                             tol, out);
+    PVL_HIP(hipGetLastError());
+    return PVL_OK;
+}
+
+constexpr int kChainPPL = 2;
+
+unsigned chain_velocity_chunks(unsigned n_pts, unsigned n_pieces)
+{
+    // aim at >= ~2048 work items while keeping >= 8 pieces (>= 2 per wave) per chunk
+    const unsigned ppg = kWave * kChainPPL;
+    const unsigned pblocks = (n_pts + ppg - 1) / ppg;
+    unsigned n_chunks = (2048 + pblocks - 1) / pblocks;
+    const unsigned max_chunks = (n_pieces + 7) / 8;
+    if (n_chunks > max_chunks)
+        n_chunks = max_chunks;
+    if (n_chunks < 1)
+        n_chunks = 1;
+    const unsigned ppc = (n_pieces + n_chunks - 1) / n_chunks;
+    return ppc ? (n_pieces + ppc - 1) / ppc : 1;
+}
+
+int chain_velocity_partials(unsigned n_pts, const double *pts, const double *nodes, const ChainPiece *pieces,
+                            unsigned n_pieces, unsigned cr, const SegRec *segs, int precision, double tol,
+                            double work, hipStream_t s, double *partial, unsigned max_groups)
+{
+    const unsigned n_chunks = chain_velocity_chunks(n_pts, n_pieces);
+    const unsigned ppc = (n_pieces + n_chunks - 1) / n_chunks;
+    const unsigned ppg = kWave * kChainPPL;
+    const unsigned pblocks = (n_pts + ppg - 1) / ppg;
+    const unsigned n_items = pblocks * n_chunks;
+    const unsigned grid = max_groups && max_groups < n_items ? max_groups : n_items;
+    ProfScope prof("wake_self_velocity", work, s);
+    dispatch_precision(precision, [&](auto P) {
+        chain_velocity_kernel<decltype(P)::value, kChainPPL><<<grid, kBlock, 0, s>>>(
+            n_pts, pts, nodes, pieces, n_pieces, cr, segs, ppc, partial, tol, pblocks, n_items);
     });
     PVL_HIP(hipGetLastError());
     return PVL_OK;

@@ -17,7 +17,8 @@
 //   PVL_FP64      one angle: atan2(t2, h) - atan2(t1, h) = atan2((t2 - t1) h, h^2 + t1 t2).
 //                 Both sides lie in (-pi, pi): with r_i = sqrt(h^2 + t_i^2) > 0, the sine and
 //                 cosine of the difference are (t2 - t1) h / (r1 r2) and (h^2 + t1 t2) /
-//                 (r1 r2).  The error of this form is O(eps) relative to the angle everywhere,
+//                 (r1 r2).  The error of this form is a few 1e-15 relative to the angle
+//                 everywhere (the atan polynomial, the quotient and 1/h below, each ~1e-15),
 //                 whereas the difference of two atan2 values loses digits in the far field
 //                 (|angle| << pi); the two forms agree to that loss (~1e-14 at r/len ~ 100).
 //                 atan2 is this library's own (pvl_atan2 below), 1/h comes from a
@@ -33,31 +34,30 @@ enum
     PVL_FP64_REF_FORM = 2
 };
 
-// atan(a) = a + a s P(s), s = a^2, 0 <= a <= 1.  Minimax coefficients (maximum relative
-// error of the polynomial 2.8e-17 for FP64, 1.7e-8 for FP32) fitted by
-// pyvl_amd/csrc/tools/fit_atan.py (Lawson iteration in 60-digit arithmetic).
+// atan(a) = a + a s P(s), s = a^2, 0 <= a <= 1.  Minimax coefficients fitted by
+// pyvl_amd/csrc/tools/fit_atan.py (Lawson iteration in 60-digit arithmetic): P of degree 16
+// for FP64 (maximum relative error of the polynomial 1.1e-15, ~5 ulp: the pair's other
+// rounding errors are of that order), degree 7 for FP32 (1.7e-8).
 __device__ __forceinline__ double atan_poly64(double a)
 {
     const double s = a * a;
-    double p = -2.0258217196429153806e-5;
-    p = fma(p, s, 2.2301921440372192015e-4);
-    p = fma(p, s, -1.1640578327914545487e-3);
-    p = fma(p, s, 3.855937704218664755e-3);
-    p = fma(p, s, -9.1844913587584364515e-3);
-    p = fma(p, s, 1.6977946435648326745e-2);
-    p = fma(p, s, -2.5826708868270838343e-2);
-    p = fma(p, s, 3.4067745269857893501e-2);
-    p = fma(p, s, -4.0926344607047341725e-2);
-    p = fma(p, s, 4.6739479480120956234e-2);
-    p = fma(p, s, -5.2392324387842287365e-2);
-    p = fma(p, s, 5.877307626399871717e-2);
-    p = fma(p, s, -6.6658603353628207695e-2);
-    p = fma(p, s, 7.6922129266775500417e-2);
-    p = fma(p, s, -9.0909012350179866105e-2);
-    p = fma(p, s, 1.1111110678724637807e-1);
-    p = fma(p, s, -1.4285714271333841475e-1);
-    p = fma(p, s, 1.999999999975499914e-1);
-    p = fma(p, s, -3.3333333333331858665e-1);
+    double p = -4.7926633120228785553e-5;
+    p = fma(p, s, 4.800660567800612957e-4);
+    p = fma(p, s, -2.2725728333658927666e-3);
+    p = fma(p, s, 6.8183912427924839848e-3);
+    p = fma(p, s, -1.4739706399322470277e-2);
+    p = fma(p, s, 2.4909137877934616446e-2);
+    p = fma(p, s, -3.5142229350564300719e-2);
+    p = fma(p, s, 4.3938991765738211806e-2);
+    p = fma(p, s, -5.1356486201903778769e-2);
+    p = fma(p, s, 5.8483233451193392607e-2);
+    p = fma(p, s, -6.6598275562737410628e-2);
+    p = fma(p, s, 7.6913027337110721971e-2);
+    p = fma(p, s, -9.090805442522420044e-2);
+    p = fma(p, s, 1.1111104041852547122e-1);
+    p = fma(p, s, -1.4285713995249626285e-1);
+    p = fma(p, s, 1.9999999993898812193e-1);
+    p = fma(p, s, -3.3333333333288106455e-1);
     return fma(a * s, p, a);
 }
 
@@ -76,21 +76,36 @@ __device__ __forceinline__ float atan_poly32(float a)
 }
 
 // atan2(y, x) for finite arguments not both zero: a = min(|x|,|y|) / max(|x|,|y|) in [0, 1]
-// (the hardware reciprocal, within 2^-24 relative on gfx950 (measured), one Newton step to
-// ~2^-48, then the quotient corrected by its residual: within about an ulp), atan(a) from
-// the polynomial, then the octant: pi/2 - . when |y| > |x|, pi - . when x < 0, sign of y.
+// (the hardware reciprocal, within 2^-24 relative on gfx950 (measured), refined by one
+// Newton step to ~2^-48: a within ~4e-15 relative, and atan(a) within that as well since
+// a / ((1 + a^2) atan(a)) <= 1), atan(a) from the polynomial, then the octant:
+// pi/2 - . when |y| > |x|, pi - . when x < 0, sign of y.
 __device__ __forceinline__ double pvl_atan2(double y, double x)
 {
     const double ax = fabs(x), ay = fabs(y);
     const double mn = fmin(ax, ay), mx = fmax(ax, ay);
     double r = __builtin_amdgcn_rcp(mx);
     r = fma(r, fma(-mx, r, 1.0), r);
-    double a = mn * r;
-    a = fma(r, fma(-mx, a, mn), a);
+    const double a = mn * r;
     double t = atan_poly64(a);
     t = (ay > ax) ? (1.57079632679489661923 - t) : t;
     t = (x < 0.0) ? (3.14159265358979323846 - t) : t;
     return copysign(t, y);
+}
+
+// atan2(-ay, x) for ay >= 0 (the segment's angle below: its sine is never positive), as
+// pvl_atan2 without the sign transfer.
+__device__ __forceinline__ double pvl_atan2_neg(double ay, double x)
+{
+    const double ax = fabs(x);
+    const double mn = fmin(ax, ay), mx = fmax(ax, ay);
+    double r = __builtin_amdgcn_rcp(mx);
+    r = fma(r, fma(-mx, r, 1.0), r);
+    const double a = mn * r;
+    double t = atan_poly64(a);
+    t = (ay > ax) ? (1.57079632679489661923 - t) : t;
+    t = (x < 0.0) ? (3.14159265358979323846 - t) : t;
+    return -t;
 }
 
 __device__ __forceinline__ float pvl_atan2f(float y, float x)
@@ -115,6 +130,57 @@ __device__ __forceinline__ double pvl_rsqrt(double x)
     return y;
 }
 
+// Scalar factor of K from the pair's endpoint terms: dr1 = p - r1 and q1 = |dr1|^2, dr2 and
+// q2 likewise (each dot product left to right, no FMA, as the reference), e = the unit
+// direction.  The cut-off is decided here.  Split from line_m so that a walk along a chain of
+// segments can reuse one endpoint's terms for the next segment (bit-identical values).
+template <int PREC>
+__device__ __forceinline__ double pair_m(double ex, double ey, double ez, double len, double d1x, double d1y,
+                                         double d1z, double q1, double d2x, double d2y, double d2z, double q2,
+                                         double tol, double tol2, bool &cut)
+{
+    const double t1 = ex * d1x + ey * d1y + ez * d1z;
+    const double t2 = ex * d2x + ey * d2y + ez * d2z;
+    const double n1 = q1 - t1 * t1;
+    const double n2 = q2 - t2 * t2;
+    const double h2 = (n1 + n2) / 2.0;
+    double m;
+    if constexpr (PREC == PVL_FP64_REF_FORM)
+    {
+        cut = (len < tol) || (h2 < tol2);
+        const double hh = cut ? 1.0 : h2;
+        const double h = sqrt(hh);
+        m = (atan2(t2, h) - atan2(t1, h)) / h;
+    }
+    else
+    {
+        // Only h2 < tol^2 is tested here: a line shorter than tol has direction 0 and length 0
+        // (line_geom_kernel), so t1 = t2 = 0, the angle atan2(-0, h2 > 0) is 0 and the line
+        // adds exactly 0, as the reference's cut-off does.  A cut pair's h2 (possibly <= 0)
+        // may give NaN or Inf below; the caller replaces the pair's value by 0.
+        // t2 - t1 = e . (r1 - r2) = -len exactly in exact arithmetic; -len is used (the
+        // rounded difference of t1 and t2 carries an error of eps |dr| / len relative in the
+        // far field).  A line cut by length has len = 0 (line_geom_kernel), so its angle is 0.
+        cut = h2 < tol2;
+        if constexpr (PREC == PVL_FP64)
+        {
+            const double ih = pvl_rsqrt(h2);
+            const double h = h2 * ih;
+            const double dphi = pvl_atan2_neg(len * h, fma(t1, t2, h2));
+            m = dphi * ih;
+        }
+        else
+        {
+            const float hf = sqrtf((float)h2);
+            const float ihf = __builtin_amdgcn_rcpf(hf);
+            const float num = -(float)len * hf;
+            const float den = (float)fma(t1, t2, h2);
+            m = (double)(pvl_atan2f(num, den) * ihf);
+        }
+    }
+    return m;
+}
+
 // Scalar factor of K (angle / h; the cut-off is decided here) and dr1 = p - r1.
 template <int PREC>
 __device__ __forceinline__ double line_m(const LineGeom &g, double px, double py, double pz, double tol, double tol2,
@@ -122,43 +188,9 @@ __device__ __forceinline__ double line_m(const LineGeom &g, double px, double py
 {
     d1x = px - g.r1x, d1y = py - g.r1y, d1z = pz - g.r1z;
     const double d2x = px - g.r2x, d2y = py - g.r2y, d2z = pz - g.r2z;
-    const double t1 = g.ex * d1x + g.ey * d1y + g.ez * d1z;
-    const double t2 = g.ex * d2x + g.ey * d2y + g.ez * d2z;
-    const double n1 = (d1x * d1x + d1y * d1y + d1z * d1z) - t1 * t1;
-    const double n2 = (d2x * d2x + d2y * d2y + d2z * d2z) - t2 * t2;
-    const double h2 = (n1 + n2) / 2.0;
-    double m;
-    if constexpr (PREC == PVL_FP64_REF_FORM)
-    {
-        cut = (g.len < tol) || (h2 < tol2);
-        const double hh = cut ? 1.0 : h2;
-        const double h = sqrt(hh);
-        m = (atan2(t2, h) - atan2(t1, h)) / h;
-    }
-    else
-    {
-        // Only h2 < tol^2 is tested here: a line shorter than tol has direction 0
-        // (line_geom_kernel), so t1 = t2 = 0, the angle atan2(0, h2 > 0) is 0 and the line
-        // adds exactly 0, as the reference's cut-off does.  A cut pair's h2 (possibly <= 0)
-        // may give NaN or Inf below; the caller replaces the pair's value by 0.
-        cut = h2 < tol2;
-        if constexpr (PREC == PVL_FP64)
-        {
-            const double ih = pvl_rsqrt(h2);
-            const double h = h2 * ih;
-            const double dphi = pvl_atan2((t2 - t1) * h, fma(t1, t2, h2));
-            m = dphi * ih;
-        }
-        else
-        {
-            const float hf = sqrtf((float)h2);
-            const float ihf = __builtin_amdgcn_rcpf(hf);
-            const float num = (float)(t2 - t1) * hf;
-            const float den = (float)fma(t1, t2, h2);
-            m = (double)(pvl_atan2f(num, den) * ihf);
-        }
-    }
-    return m;
+    const double q1 = d1x * d1x + d1y * d1y + d1z * d1z;
+    const double q2 = d2x * d2x + d2y * d2y + d2z * d2z;
+    return pair_m<PREC>(g.ex, g.ey, g.ez, g.len, d1x, d1y, d1z, q1, d2x, d2y, d2z, q2, tol, tol2, cut);
 }
 
 // K(line, p).  The reference form keeps the reference's cross product and products; the

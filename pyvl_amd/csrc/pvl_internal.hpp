@@ -47,7 +47,7 @@ void gmres_release_workspace();
 void set_error(const char *fmt, ...);
 hipStream_t stream();
 hipStream_t side_stream(); // nullptr (error set) if it cannot be created
-int side_groups();         // workgroup cap for grid-stride kernels on the side stream
+int side_groups(int per_cu = 5); // workgroup cap for grid-stride kernels on the side stream
 
 // Kernels on the library stream's critical path raise their waves' issue priority above
 // the side stream's (priority 0), which shares their SIMDs.
@@ -111,6 +111,39 @@ int point_velocity_partials(const pvl_mesh *m, double tol, const double *pos, co
 int point_velocity_reduce(unsigned n_pts, unsigned n_chunks, const double *partial, double *out, int accumulate,
                           hipStream_t s);
 
+// Point-velocity sum over segment chains (the wake's self-induction).  A chain is a polyline
+// of nodes; a piece is a run of up to kChainPiece consecutive segments of one chain, starting
+// at node start_node, whose segment records are segs[first_seg .. first_seg + n_seg).  A
+// piece with row1 > 0 runs streamwise from node row row1 - 1 and only its first cr - (row1 - 1)
+// segments are active at cr filled rows.  Walking a piece, each node's p - r and |p - r|^2
+// are formed once and serve both segments that meet there.
+constexpr unsigned kChainPiece = 32;
+struct ChainPiece
+{
+    uint32_t start_node, first_seg, n_seg, row1;
+};
+// Per-step segment record: end node, unit direction and length (the reference's line
+// prologue, src/core/flow_solver.c:95-103, in the chain's direction) and circulation.
+struct alignas(16) SegRec
+{
+    double nx, ny, nz;
+    double ex, ey, ez;
+    double len, gamma;
+};
+static_assert(sizeof(SegRec) == 64, "SegRec layout");
+// Line circulations gamma_l = sum of +-Gamma over the surfaces holding line l, lines < n_active.
+int line_circulations(const pvl_mesh *m, unsigned n_active, const double *circ, double *gamma, hipStream_t s);
+// SegRec of n_seg segments (seg_ab: node pairs; seg_line: mesh line | bit 31 when the chain runs
+// against the line's direction); lines >= n_active_lines carry no circulation.
+int chain_segments(unsigned n_seg, const uint32_t *seg_ab, const uint32_t *seg_line, const double *nodes,
+                   const double *gamma, unsigned n_active_lines, double tol, SegRec *out, hipStream_t s);
+unsigned chain_velocity_chunks(unsigned n_pts, unsigned n_pieces);
+// Partial sums per piece chunk (3 n_pts doubles each, in chunk order; point_velocity_reduce adds
+// them); work = pairs evaluated, for the profiler.
+int chain_velocity_partials(unsigned n_pts, const double *pts, const double *nodes, const ChainPiece *pieces,
+                            unsigned n_pieces, unsigned cr, const SegRec *segs, int precision, double tol,
+                            double work, hipStream_t s, double *partial, unsigned max_groups);
+
 // Single-workgroup LU panel (lu_panel_f64.hip / lu_panel_f32.hip): factors the m x jb panel
 // at P (row stride lda; matrix rows row_base..) with partial pivoting, piv[row_base + j]
 // receiving the 0-based interchange row, status[0] the first zero pivot (1-based column).
@@ -119,6 +152,30 @@ constexpr int panel_wg_rows(size_t elem_size) { return elem_size == 8 ? 1024 : 2
 int panel_wg(double *P, int lda, int m, int jb, int row_base, int *piv, int *status);
 int panel_wg(float *P, int lda, int m, int jb, int row_base, int *piv, int *status);
 } // namespace pvl
+
+// Free-stream velocity field on the device (python/pyvl/flow_conditions.py): uniform
+// (kind 0, velocity a) or rotating about a point (kind 1, v = (p - a) x b,
+// FlowConditionsRotating.get_velocity :412-446 with numpy's cross product order).
+struct FlowField
+{
+    int kind;
+    double a[3], b[3];
+};
+// From the C ABI's 7-double descriptor {kind, a[3], b[3]}; PVL_ERR_ARG on an unknown kind.
+int flow_from_descriptor(const double *desc, FlowField *out);
+__device__ __forceinline__ void flow_velocity(const FlowField &f, double x, double y, double z, double &vx,
+                                              double &vy, double &vz)
+{
+    if (f.kind == 0)
+    {
+        vx = f.a[0], vy = f.a[1], vz = f.a[2];
+        return;
+    }
+    const double px = x - f.a[0], py = y - f.a[1], pz = z - f.a[2];
+    vx = py * f.b[2] - pz * f.b[1];
+    vy = pz * f.b[0] - px * f.b[2];
+    vz = px * f.b[1] - py * f.b[0];
+}
 
 // Line geometry precomputed once per call: start point, end point, unit direction and
 // length, exactly as the reference's per-line prologue (src/core/flow_solver.c:95-103).

@@ -86,23 +86,22 @@ int ensure_init()
 hipStream_t stream() { return g_stream; }
 
 // Workgroups per CU given to work on the side stream (environment PVL_SIDE_GROUPS_PER_CU,
-// default 5): the rest of each CU's slots stay free for the library stream's kernels.
-int side_groups()
+// default per_cu): the rest of each CU's slots stay free for the library stream's kernels.
+int side_groups(int per_cu)
 {
-    static int groups = -1;
-    if (groups < 0)
+    static int cus = -1;
+    if (cus < 0)
     {
-        int per_cu = 5;
-        if (const char *e = getenv("PVL_SIDE_GROUPS_PER_CU"))
-            per_cu = atoi(e) > 0 ? atoi(e) : per_cu;
-        int cus = 0;
-        if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, g_device < 0 ? 0 : g_device) !=
+        int n = 0;
+        if (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, g_device < 0 ? 0 : g_device) !=
                 hipSuccess ||
-            cus <= 0)
-            cus = 256;
-        groups = per_cu * cus;
+            n <= 0)
+            n = 256;
+        cus = n;
     }
-    return groups;
+    if (const char *e = getenv("PVL_SIDE_GROUPS_PER_CU"))
+        per_cu = atoi(e) > 0 ? atoi(e) : per_cu;
+    return per_cu * cus;
 }
 
 static bool cooperative_launches()

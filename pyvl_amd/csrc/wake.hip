@@ -24,6 +24,7 @@
 //   b += n . sum Gamma K.
 #include "pvl_internal.hpp"
 
+#include <algorithm>
 #include <unordered_map>
 #include <vector>
 
@@ -55,6 +56,16 @@ struct pvl_wake
     double *d_spartial = nullptr;      // [3 * n_pts * chunks]
     size_t spartial_bytes = 0;
     hipEvent_t ev_ready = nullptr, ev_done = nullptr;
+    // The self-induction's segments as chains over the distinct nodes (chain_velocity_kernel):
+    // streamwise chains per trailing-edge column, spanwise chains per row along runs of
+    // adjacent strips, cut into pieces ordered by the ring row that activates them.
+    bool chains = false;
+    unsigned n_seg = 0;
+    uint32_t *d_seg_ab = nullptr;   // [2 n_seg] node pairs, chain direction
+    uint32_t *d_seg_line = nullptr; // [n_seg] line of `full` | kReversed when against it
+    ChainPiece *d_pieces = nullptr;
+    std::vector<uint32_t> pieces_at; // [R+1] pieces active with r ring rows filled
+    SegRec *d_segrec = nullptr;      // [n_seg] per-step records
     bool pending = false;
     int pending_prec = 0;
     unsigned pending_na = 0, pending_npts = 0, pending_chunks = 0;
@@ -76,10 +87,10 @@ __global__ void pin_row0_kernel(double *__restrict__ upos, unsigned n_te, const 
     upos[3 * i + 2] = bound_pos[3 * p + 2];
 }
 
-// upos[r+1] = old[r] + (V_inf + vel[r]) * dt for rows r < cr (python/pyvl/wake_models.py:121-128)
+// upos[r+1] = old[r] + (V_inf(old[r]) + vel[r]) * dt for rows r < cr
+// (python/pyvl/wake_models.py:121-128: the free stream at the node's position before the step)
 __global__ void advect_kernel(double *__restrict__ upos, const double *__restrict__ old,
-                              const double *__restrict__ vel, unsigned n, unsigned n_te, double vx, double vy,
-                              double vz, double dt)
+                              const double *__restrict__ vel, unsigned n, unsigned n_te, FlowField flow, double dt)
 {
     main_stream_priority();
     const unsigned i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -87,6 +98,8 @@ __global__ void advect_kernel(double *__restrict__ upos, const double *__restric
         return;
     const double *o = old + 3 * i, *v = vel + 3 * i;
     double *d = upos + 3 * ((size_t)n_te + i);
+    double vx, vy, vz;
+    flow_velocity(flow, o[0], o[1], o[2], vx, vy, vz);
     d[0] = o[0] + (vx + v[0]) * dt;
     d[1] = o[1] + (vy + v[1]) * dt;
     d[2] = o[2] + (vz + v[2]) * dt;
@@ -231,6 +244,128 @@ int build_topology(pvl_wake *w, std::vector<uint32_t> &lines, std::vector<uint32
     return PVL_OK;
 }
 
+// The chains of the wake's lines (see pvl_wake::chains).  Every line of `full` must be one
+// segment of one chain; otherwise (unusual strip topologies) the generic line sum is used.
+int build_chains(pvl_wake *w, const std::vector<uint32_t> &lines)
+{
+    const unsigned R = w->R, n_te = w->n_te, n_shed = w->n_shed;
+    const size_t n_lines = lines.size() / 2;
+    std::unordered_map<uint64_t, uint32_t> id;
+    id.reserve(n_lines);
+    for (size_t l = 0; l < n_lines; ++l)
+    {
+        const uint32_t a = lines[2 * l], b = lines[2 * l + 1];
+        id.emplace((uint64_t(a > b ? a : b) << 32) | (a > b ? b : a), (uint32_t)l);
+    }
+    std::vector<uint32_t> ab, sl;
+    std::vector<std::pair<uint32_t, ChainPiece>> keyed;
+    std::vector<char> used(n_lines, 0);
+    bool ok = true;
+    // nodes n_0..n_k of one chain; ring row of segment j: streamwise j, spanwise (row - 1, row)
+    auto add_chain = [&](const std::vector<uint32_t> &nodes, bool streamwise, unsigned row) {
+        const unsigned k = (unsigned)nodes.size() - 1;
+        for (unsigned q = 0; q < k; q += kChainPiece)
+        {
+            ChainPiece pc;
+            pc.start_node = nodes[q];
+            pc.first_seg = (uint32_t)sl.size();
+            pc.n_seg = (k - q) < kChainPiece ? (k - q) : kChainPiece;
+            pc.row1 = streamwise ? q + 1 : 0;
+            const uint32_t key = streamwise ? q + 1 : (row == 0 ? 1 : row);
+            keyed.push_back({key, pc});
+            for (unsigned j = q; j < q + pc.n_seg; ++j)
+            {
+                const uint32_t a = nodes[j], b = nodes[j + 1];
+                auto it = id.find((uint64_t(a > b ? a : b) << 32) | (a > b ? b : a));
+                if (it == id.end() || used[it->second])
+                {
+                    ok = false;
+                    return;
+                }
+                used[it->second] = 1;
+                const uint32_t l = it->second;
+                ab.push_back(a), ab.push_back(b);
+                sl.push_back(lines[2 * l] == a ? l : (l | kReversed));
+            }
+        }
+    };
+    // runs of adjacent strips: strip t follows strip s when its side-0 column is s's side-1 column
+    std::vector<std::vector<uint32_t>> starting(n_te);
+    std::vector<char> has_pred(n_shed, 0), seen(n_shed, 0);
+    for (unsigned s2 = 0; s2 < n_shed; ++s2)
+        starting[w->col[2 * s2]].push_back(s2);
+    for (unsigned s2 = 0; s2 < n_shed; ++s2)
+        for (uint32_t t : starting[w->col[2 * s2 + 1]])
+            if (t != s2)
+                has_pred[t] = 1;
+    std::vector<std::vector<uint32_t>> runs;
+    for (int pass = 0; pass < 2; ++pass)
+        for (unsigned s0 = 0; s0 < n_shed; ++s0)
+        {
+            if (seen[s0] || (pass == 0 && has_pred[s0]))
+                continue;
+            std::vector<uint32_t> run;
+            for (int cur = (int)s0; cur >= 0;)
+            {
+                seen[cur] = 1;
+                run.push_back((uint32_t)cur);
+                int nxt = -1;
+                for (uint32_t t : starting[w->col[2 * cur + 1]])
+                    if (!seen[t])
+                    {
+                        nxt = (int)t;
+                        break;
+                    }
+                cur = nxt;
+            }
+            runs.push_back(run);
+        }
+    for (unsigned r = 0; r <= R && ok; ++r)
+        for (const auto &run : runs)
+        {
+            std::vector<uint32_t> nodes{r * n_te + w->col[2 * run[0]]};
+            for (uint32_t s2 : run)
+                nodes.push_back(r * n_te + w->col[2 * s2 + 1]);
+            add_chain(nodes, false, r);
+            if (!ok)
+                break;
+        }
+    for (unsigned c = 0; c < n_te && ok; ++c)
+    {
+        std::vector<uint32_t> nodes(R + 1);
+        for (unsigned r = 0; r <= R; ++r)
+            nodes[r] = r * n_te + c;
+        add_chain(nodes, true, 0);
+    }
+    if (!ok || sl.size() != n_lines)
+        return PVL_OK; // generic path
+    std::stable_sort(keyed.begin(), keyed.end(),
+                     [](const std::pair<uint32_t, ChainPiece> &x, const std::pair<uint32_t, ChainPiece> &y) {
+                         return x.first < y.first;
+                     });
+    std::vector<ChainPiece> pieces(keyed.size());
+    w->pieces_at.assign(R + 1, 0);
+    for (size_t i = 0; i < keyed.size(); ++i)
+        pieces[i] = keyed[i].second;
+    for (unsigned rr = 0; rr <= R; ++rr)
+    {
+        uint32_t cnt = 0;
+        while (cnt < keyed.size() && keyed[cnt].first <= rr)
+            ++cnt;
+        w->pieces_at[rr] = cnt;
+    }
+    w->n_seg = (unsigned)sl.size();
+    PVL_HIP(hipMalloc((void **)&w->d_seg_ab, sizeof(uint32_t) * ab.size()));
+    PVL_HIP(hipMalloc((void **)&w->d_seg_line, sizeof(uint32_t) * sl.size()));
+    PVL_HIP(hipMalloc((void **)&w->d_pieces, sizeof(ChainPiece) * pieces.size()));
+    PVL_HIP(hipMalloc((void **)&w->d_segrec, sizeof(SegRec) * sl.size()));
+    PVL_HIP(hipMemcpy(w->d_seg_ab, ab.data(), sizeof(uint32_t) * ab.size(), hipMemcpyHostToDevice));
+    PVL_HIP(hipMemcpy(w->d_seg_line, sl.data(), sizeof(uint32_t) * sl.size(), hipMemcpyHostToDevice));
+    PVL_HIP(hipMemcpy(w->d_pieces, pieces.data(), sizeof(ChainPiece) * pieces.size(), hipMemcpyHostToDevice));
+    w->chains = true;
+    return PVL_OK;
+}
+
 } // namespace
 
 extern "C" {
@@ -280,6 +415,8 @@ int pvl_wake_create(pvl_wake **out, unsigned n_shed, unsigned line_rows, const u
     if (!st)
         st = pvl_mesh_create(&w->full, (R + 1) * n_te, (unsigned)(lines.size() / 2), lines.data(), R * n_shed,
                              off.data(), ent.data());
+    if (!st)
+        st = build_chains(w, lines);
     if (!st)
     {
         std::vector<uint32_t> l0, o0, e0;
@@ -336,6 +473,10 @@ int pvl_wake_destroy(pvl_wake *w)
     (void)hipFree(w->d_sgeom);
     (void)hipFree(w->d_sgamma);
     (void)hipFree(w->d_spartial);
+    (void)hipFree(w->d_seg_ab);
+    (void)hipFree(w->d_seg_line);
+    (void)hipFree(w->d_pieces);
+    (void)hipFree(w->d_segrec);
     pvl_mesh_destroy(w->full);
     pvl_mesh_destroy(w->row0);
     (void)hipFree(w->d_te_nodes);
@@ -364,15 +505,64 @@ int pvl_wake_info(const pvl_wake *w, unsigned *n_te, unsigned *n_lines, unsigned
     return PVL_OK;
 }
 
-// Lines carrying circulation: ring rows < min(step_count - 1, R).
-static unsigned active_lines(const pvl_wake *w)
+// Ring rows that may carry circulation: min(step_count - 1, R).
+static unsigned active_rows(const pvl_wake *w)
 {
     if (w->step_count <= 1)
         return 0;
-    unsigned rows = w->step_count - 1;
-    if (rows > w->R)
-        rows = w->R;
-    return w->nl_prefix[rows];
+    return w->step_count - 1 < w->R ? w->step_count - 1 : w->R;
+}
+
+// Lines carrying circulation: those of ring rows < active_rows.
+static unsigned active_lines(const pvl_wake *w) { return w->nl_prefix[active_rows(w)]; }
+
+static unsigned self_chunks(const pvl_wake *w, unsigned n_pts)
+{
+    if (w->chains)
+        return chain_velocity_chunks(n_pts, w->pieces_at[active_rows(w)]);
+    return point_velocity_chunks(n_pts, active_lines(w), "wake_self_velocity");
+}
+
+// Buffers of the self-induction: line circulations, segment records (chains) or line
+// geometry, and the per-chunk partial sums (grown on demand; a started sum must have been
+// consumed or dropped before they are regrown).
+static int self_buffers(pvl_wake *w, unsigned n_pts)
+{
+    if (!w->chains && !w->d_sgeom)
+        PVL_HIP(hipMalloc(&w->d_sgeom, line_geom_bytes(w->full->n_lines)));
+    if (!w->d_sgamma)
+        PVL_HIP(hipMalloc((void **)&w->d_sgamma, sizeof(double) * (w->full->n_lines ? w->full->n_lines : 1)));
+    const size_t pbytes = sizeof(double) * 3 * (size_t)n_pts * self_chunks(w, n_pts);
+    if (pbytes > w->spartial_bytes)
+    {
+        // no queued kernel on either library stream may still use the old buffer
+        PVL_HIP(hipStreamSynchronize(stream()));
+        if (hipStream_t s2 = side_stream())
+            PVL_HIP(hipStreamSynchronize(s2));
+        (void)hipFree(w->d_spartial);
+        w->d_spartial = nullptr, w->spartial_bytes = 0;
+        PVL_HIP(hipMalloc((void **)&w->d_spartial, pbytes));
+        w->spartial_bytes = pbytes;
+    }
+    return PVL_OK;
+}
+
+// Partial sums of the wake's own induction at the n_pts points pts (the first n_pts wake
+// nodes), from the state at this moment, on stream s with at most max_groups workgroups.
+static int self_partials(pvl_wake *w, unsigned n_pts, const double *pts, int precision, hipStream_t s,
+                         unsigned max_groups)
+{
+    const unsigned ar = active_rows(w), na = w->nl_prefix[ar];
+    if (!w->chains)
+        return point_velocity_partials(w->full, w->tol, w->d_upos, w->d_circ[w->cur], n_pts, pts, precision, na,
+                                       "wake_self_velocity", s, w->d_sgeom, w->d_sgamma, w->d_spartial, max_groups);
+    if (int st = line_circulations(w->full, na, w->d_circ[w->cur], w->d_sgamma, s))
+        return st;
+    if (int st = chain_segments(w->n_seg, w->d_seg_ab, w->d_seg_line, w->d_upos, w->d_sgamma, na, w->tol, w->d_segrec,
+                                s))
+        return st;
+    return chain_velocity_partials(n_pts, pts, w->d_upos, w->d_pieces, w->pieces_at[ar], ar, w->d_segrec, precision,
+                                   w->tol, (double)n_pts * (double)na, s, w->d_spartial, max_groups);
 }
 
 // Orders the library stream after a started self-induction (its inputs may be overwritten
@@ -413,27 +603,15 @@ int pvl_wake_begin_update(pvl_wake *w, int mode, int precision)
         PVL_HIP(hipEventCreateWithFlags(&w->ev_ready, hipEventDisableTiming));
     if (!w->ev_done)
         PVL_HIP(hipEventCreateWithFlags(&w->ev_done, hipEventDisableTiming));
-    if (!w->d_sgeom)
-        PVL_HIP(hipMalloc(&w->d_sgeom, line_geom_bytes(w->full->n_lines)));
-    if (!w->d_sgamma)
-        PVL_HIP(hipMalloc((void **)&w->d_sgamma, sizeof(double) * (w->full->n_lines ? w->full->n_lines : 1)));
-    const unsigned nc = point_velocity_chunks(n_old, na, "wake_self_velocity");
-    const size_t pbytes = sizeof(double) * 3 * (size_t)n_old * nc;
-    if (pbytes > w->spartial_bytes)
-    {
-        // the previous buffer is no longer read: every started sum was consumed or dropped
-        (void)hipFree(w->d_spartial);
-        w->d_spartial = nullptr, w->spartial_bytes = 0;
-        PVL_HIP(hipMalloc((void **)&w->d_spartial, pbytes));
-        w->spartial_bytes = pbytes;
-    }
+    // no started sum is pending here: the buffers are free to regrow
+    if (int st = self_buffers(w, n_old))
+        return st;
+    const unsigned nc = self_chunks(w, n_old);
     // the wake state as the library stream leaves it now; the points are the first n_old
     // wake nodes (pvl_wake_update evaluates at a copy of the same values)
     PVL_HIP(hipEventRecord(w->ev_ready, stream()));
     PVL_HIP(hipStreamWaitEvent(s, w->ev_ready, 0));
-    if (int st = point_velocity_partials(w->full, w->tol, w->d_upos, w->d_circ[w->cur], n_old, w->d_upos, precision,
-                                         na, "wake_self_velocity", s, w->d_sgeom, w->d_sgamma, w->d_spartial,
-                                         (unsigned)side_groups()))
+    if (int st = self_partials(w, n_old, w->d_upos, precision, s, (unsigned)side_groups(w->chains ? 4 : 5)))
         return st;
     PVL_HIP(hipEventRecord(w->ev_done, s));
     w->pending = true;
@@ -444,7 +622,17 @@ int pvl_wake_begin_update(pvl_wake *w, int mode, int precision)
 int pvl_wake_update(pvl_wake *w, double time, const pvl_mesh *bound, const double *bound_pos_dev,
                     const double *circulation_dev, const double *vinf, int mode, int precision)
 {
+    const double desc[7] = {0.0, vinf[0], vinf[1], vinf[2], 0.0, 0.0, 0.0};
+    return pvl_wake_update_flow(w, time, bound, bound_pos_dev, circulation_dev, desc, mode, precision);
+}
+
+int pvl_wake_update_flow(pvl_wake *w, double time, const pvl_mesh *bound, const double *bound_pos_dev,
+                         const double *circulation_dev, const double *flow, int mode, int precision)
+{
     PVL_API_LOCK;
+    FlowField fl;
+    if (int st = flow_from_descriptor(flow, &fl))
+        return st;
     if (int st = ensure_init())
         return st;
     if (mode != 0 && mode != 1)
@@ -485,15 +673,17 @@ int pvl_wake_update(pvl_wake *w, double time, const pvl_mesh *bound, const doubl
         {
             if (int st = drop_pending(w))
                 return st;
-            if (int st = point_velocity_impl(w->full, w->tol, w->d_upos, w->d_circ[w->cur], n_old, w->d_old, w->d_vel,
-                                             1, precision, na, "wake_self_velocity"))
+            if (int st = self_buffers(w, n_old))
+                return st;
+            if (int st = self_partials(w, n_old, w->d_old, precision, stream(), 0))
+                return st;
+            if (int st = point_velocity_reduce(n_old, self_chunks(w, n_old), w->d_spartial, w->d_vel, 1, stream()))
                 return st;
         }
     }
     if (int st = drop_pending(w))
         return st;
-    advect_kernel<<<(n_old + 255) / 256, 256, 0, stream()>>>(w->d_upos, w->d_old, w->d_vel, n_old, n_te, vinf[0],
-                                                             vinf[1], vinf[2], dt);
+    advect_kernel<<<(n_old + 255) / 256, 256, 0, stream()>>>(w->d_upos, w->d_old, w->d_vel, n_old, n_te, fl, dt);
     PVL_HIP(hipGetLastError());
     pin_row0_kernel<<<(n_te + 255) / 256, 256, 0, stream()>>>(w->d_upos, n_te, bound_pos_dev, w->d_te_nodes);
     PVL_HIP(hipGetLastError());

@@ -148,11 +148,12 @@ class Mesh:
     would be read out of bounds.
     """
 
-    __slots__ = ("_n_points", "_lines", "_offsets", "_entries", "_dev", "__weakref__")
+    __slots__ = ("_n_points", "_lines", "_offsets", "_entries", "_dev", "_dev_lines", "__weakref__")
 
     def __new__(cls, n_points: int, connectivity):
         self = object.__new__(cls)
         self._dev = None
+        self._dev_lines = None
         n_points = int(n_points)
         if n_points < 0 or n_points > 0xFFFFFFFF:
             raise OverflowError("n_points out of range for an unsigned int")
@@ -211,6 +212,7 @@ class Mesh:
     def _from_arrays(cls, n_points: int, lines, offsets, entries) -> "Mesh":
         self = object.__new__(cls)
         self._dev = None
+        self._dev_lines = None
         self._n_points = int(n_points)
         self._lines = np.ascontiguousarray(lines, np.uint32).reshape(-1, 2)
         self._offsets = np.ascontiguousarray(offsets, np.uint32)
@@ -399,6 +401,19 @@ class Mesh:
         o[...] = (v[a] + v[b]) * 0.5
 
     # -- device mesh ----------------------------------------------------------------------
+    def _device_lines(self) -> DeviceArray:
+        """The line connectivity alone in device memory (uint32 pairs, INVALID_ID allowed),
+        uploaded once: what Mesh.line_forces reads of a dual mesh, which pvl_mesh_create
+        (points must exist) would not accept."""
+        d = self._dev_lines
+        if d is None:
+            ld = np.ascontiguousarray(self.line_data.reshape(-1), np.uint32)
+            d = DeviceArray((max(ld.size, 1),), np.uint32)
+            if ld.size:
+                d.copy_from(ld)
+            self._dev_lines = d
+        return d
+
     def _device(self):
         if self._dev is None:
             h = C.c_void_p()
@@ -615,6 +630,51 @@ class Mesh:
         q = DeviceArray.from_numpy(np.asarray(points, np.float64))
         o = DeviceArray.from_numpy(out) if (out is not None and accumulate) else None
         r = self.point_velocity(tol, p, c, q, o, accumulate=accumulate, precision=precision)
+        if out is None:
+            return r.numpy()
+        r.numpy(out)
+        return out
+
+    @staticmethod
+    def line_forces(primal: "Mesh", dual: "Mesh", circulation, positions, freestream, out=None):
+        """Forces on the primal lines from their (reduced) line circulation in the mean velocity
+        of their end points (src/meshobject.c:1288-1396): for line (p1, p2) whose dual line is
+        (s+, s-), c = circulation[s+] - circulation[s-] (INVALID_ID ends skipped) and
+        f = (r_p2 - r_p1) x ((v_p1 + v_p2) * (0.5 c)).  Static, as in the reference.
+
+        numpy arguments: returns (n_lines, 3) numpy (or fills ``out``).  DeviceArray
+        arguments: stays in HBM and returns a DeviceArray.
+        """
+        if not isinstance(primal, Mesh) or not isinstance(dual, Mesh):
+            raise TypeError("primal and dual must be Mesh objects")
+        n_lines = primal.n_lines
+        if primal.n_points != dual.n_surfaces or n_lines != dual.n_lines or primal.n_surfaces != dual.n_points:
+            raise ValueError(
+                "Given meshes can not be dual to each other, since the number of points,lines, and surfaces don't "
+                f"match as primal ({primal.n_points}, {n_lines}, {primal.n_surfaces}) and dual "
+                f"({dual.n_points}, {dual.n_lines}, {dual.n_surfaces}).")
+        if isinstance(positions, DeviceArray):
+            c = _ensure_device(circulation, 1, (primal.n_surfaces,), "Circulation array")
+            p = _ensure_device(positions, 2, (primal.n_points, 3), "Positions array")
+            v = _ensure_device(freestream, 2, (primal.n_points, 3), "Velocity array")
+            if out is None:
+                out = DeviceArray((n_lines, 3))
+            _ensure_device(out, 2, (n_lines, 3), "Output array")
+            _lib.call("pvl_line_forces", primal._device(), dual._device_lines().cptr, C.c_uint(dual.n_points),
+                      C.c_uint(dual.n_lines), C.c_uint(dual.n_surfaces), c.cptr, p.cptr, v.cptr, out.cptr)
+            return out
+        for a in (circulation, positions, freestream):
+            if not isinstance(a, np.ndarray):
+                raise TypeError(f"argument must be numpy.ndarray, not {type(a).__name__}")
+        if out is not None and not isinstance(out, np.ndarray):
+            raise TypeError(f"argument 'out' must be numpy.ndarray, not {type(out).__name__}")
+        c = _ensure_array(circulation, 1, (primal.n_surfaces,), "Circulation array")
+        p = _ensure_array(positions, 2, (primal.n_points, 3), "Positions array")
+        v = _ensure_array(freestream, 2, (primal.n_points, 3), "Velocity array")
+        if out is not None:
+            _ensure_array(out, 2, (n_lines, 3), "Output array", writeable=True)
+        r = Mesh.line_forces(primal, dual, DeviceArray.from_numpy(c), DeviceArray.from_numpy(p),
+                             DeviceArray.from_numpy(v))
         if out is None:
             return r.numpy()
         r.numpy(out)

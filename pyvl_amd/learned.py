@@ -239,7 +239,7 @@ class Normaliser:
 # The pitch + heave tapered-wing workload (2,048 panels, free wake): data, training, use.
 def _kinematics(frame, t):
     """Pitch angle, rate and acceleration; heave rate and acceleration."""
-    th, thd, h, hd = frame._angles(t)
+    th, thd, h, hd = frame.motion(t)
     w2 = frame.omega * frame.omega
     return float(th), float(thd), float(-w2 * th), float(hd), float(-w2 * h)
 

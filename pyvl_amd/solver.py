@@ -6,9 +6,10 @@ Per step, in the reference's order (solver.py:175-239):
   -> LU solve -> closed-body mean removal -> wake update.
 
 Kinematics plug in through the reference frames' ``at_time`` / ``add_velocity`` contract
-(src/referenceframeobject.c:700-729, :939-955); :class:`HeaveFrame` and
-:class:`PitchHeaveFrame` are closed-form frames for the oscillating-wing workloads (the
-reference has translating and rotor frames only).
+(src/referenceframeobject.c:700-729, :939-955; :mod:`pyvl_amd.reference_frames`).  Each
+step evaluates the frame chain at the step's time on the host (a few numbers per frame) and
+walks it for every point on the GPU (``pvl_frame_kinematics``), so the body's positions and
+velocities never leave HBM.
 """
 
 from __future__ import annotations
@@ -23,97 +24,11 @@ from . import _lib
 from . import linalg as LA
 from .cvl import INVALID_ID, Mesh
 from .device import DeviceArray
-from .wake_models import WakeModelLineExplicitUnsteady, line_adjacency_information
-
-
-@dataclass(frozen=True)
-class FlowConditionsUniform:
-    """Uniform free stream (python/pyvl/flow_conditions.py:201-233)."""
-
-    vx: float
-    vy: float
-    vz: float
-
-    def get_velocity(self, time, positions, out_array=None):
-        v = np.array((self.vx, self.vy, self.vz), np.float64)
-        if out_array is not None:
-            out_array[:, :] = v[None, :]
-            return out_array
-        return np.full_like(np.asarray(positions, np.float64), v)
-
-    @property
-    def vector(self) -> np.ndarray:
-        return np.array((self.vx, self.vy, self.vz), np.float64)
-
-
-@dataclass(frozen=True)
-class StaticFrame:
-    """Identity frame (``ReferenceFrame()``)."""
-
-    def positions(self, p0: np.ndarray, t: float) -> np.ndarray:
-        return p0
-
-    def velocities(self, p0: np.ndarray, t: float) -> np.ndarray:
-        return np.zeros_like(p0)
-
-
-@dataclass(frozen=True)
-class HeaveFrame:
-    """Vertical oscillation z(t) = h0 sin(omega t + phase)."""
-
-    h0: float
-    omega: float
-    phase: float = 0.0
-
-    def positions(self, p0, t):
-        out = np.array(p0, np.float64)
-        out[:, 2] += self.h0 * np.sin(self.omega * t + self.phase)
-        return out
-
-    def velocities(self, p0, t):
-        v = np.zeros_like(p0)
-        v[:, 2] = self.h0 * self.omega * np.cos(self.omega * t + self.phase)
-        return v
-
-
-@dataclass(frozen=True)
-class PitchHeaveFrame:
-    """Pitch theta(t) = theta0 sin(omega t + phase_p) about the spanwise (y) axis through
-    ``pivot_x``, plus heave h(t) = h0 sin(omega t + phase_h)."""
-
-    theta0: float
-    h0: float
-    omega: float
-    pivot_x: float = 0.25
-    phase_p: float = 0.0
-    phase_h: float = 0.0
-
-    def _angles(self, t):
-        th = self.theta0 * np.sin(self.omega * t + self.phase_p)
-        thd = self.theta0 * self.omega * np.cos(self.omega * t + self.phase_p)
-        h = self.h0 * np.sin(self.omega * t + self.phase_h)
-        hd = self.h0 * self.omega * np.cos(self.omega * t + self.phase_h)
-        return th, thd, h, hd
-
-    def positions(self, p0, t):
-        th, _, h, _ = self._angles(t)
-        c, s = np.cos(th), np.sin(th)
-        x = p0[:, 0] - self.pivot_x
-        z = p0[:, 2]
-        out = np.array(p0, np.float64)
-        out[:, 0] = self.pivot_x + c * x + s * z
-        out[:, 2] = -s * x + c * z + h
-        return out
-
-    def velocities(self, p0, t):
-        th, thd, _, hd = self._angles(t)
-        c, s = np.cos(th), np.sin(th)
-        x = p0[:, 0] - self.pivot_x
-        z = p0[:, 2]
-        v = np.zeros_like(p0)
-        v[:, 0] = thd * (-s * x + c * z)
-        v[:, 2] = thd * (-c * x - s * z) + hd
-        return v
+from .flow_conditions import FlowConditions, FlowConditionsRotating, FlowConditionsUniform  # noqa: F401
+from .reference_frames import (HeaveFrame, PitchHeaveFrame, ReferenceFrame, RotorReferenceFrame,  # noqa: F401
+                               TranslatingReferenceFrame, device_chain_terms)
+from .settings import ModelSettings, SolverSettings, TimeSettings
+from .wake_models import WakeModelLineExplicitUnsteady, WakeSnapshot, line_adjacency_information
 
 
 def is_closed(mesh: Mesh) -> bool:
@@ -142,7 +57,7 @@ class UnsteadySolver:
     FP64 iterative refinement to the relative residual ``ir_tol``).
     """
 
-    def __init__(self, mesh: Mesh, positions0: np.ndarray, frame, flow: FlowConditionsUniform, vortex_limit: float,
+    def __init__(self, mesh: Mesh, positions0: np.ndarray, frame, flow: FlowConditions, vortex_limit: float,
                  dt: float, wake: WakeModelLineExplicitUnsteady | None = None, *, induction_precision: int = 0,
                  solve: str = "lu64", ir_tol: float = 1e-13, closed: bool | None = None):
         self.mesh = mesh
@@ -157,6 +72,7 @@ class UnsteadySolver:
         self.ir_tol = ir_tol
         self.closed = is_closed(mesh) if closed is None else closed
         n, p = mesh.n_surfaces, mesh.n_points
+        self.d_p0 = DeviceArray.from_numpy(self.p0)
         self.d_pos = DeviceArray((p, 3))
         self.d_vel = DeviceArray((p, 3))
         self.d_cp = DeviceArray((n, 3))
@@ -167,7 +83,7 @@ class UnsteadySolver:
         self.d_x = DeviceArray((n,))
         # partial pivoting's factorisation; without interchanges when it makes none
         self.lu = LA.LU(n, LA.FP32 if solve == "ir32" else LA.FP64, pivoting="auto")
-        self.vinf = np.ascontiguousarray(flow.vector)
+        self.flow_desc = np.ascontiguousarray(flow.device_descriptor(), np.float64)
         self.iteration = 0
         self.ir_iters: list[int] = []
         # "gmres": FP32 GMRES in FP64 refinement from warm_start(self) (numpy (n,) or None
@@ -185,8 +101,10 @@ class UnsteadySolver:
         self.d_A32 = DeviceArray((n, n), np.float32) if solve == "gmres" else None
 
     def _kinematics(self, t: float) -> None:
-        self.d_pos.copy_from(self.frame.positions(self.p0, t))
-        self.d_vel.copy_from(self.frame.velocities(self.p0, t))
+        terms, kinds = device_chain_terms(self.frame.at_time(t))
+        _lib.call("pvl_frame_kinematics", C.c_uint(self.p0.shape[0]), self.d_p0.cptr, C.c_uint(kinds.size),
+                  terms.ctypes.data_as(C.POINTER(C.c_double)), kinds.ctypes.data_as(C.POINTER(C.c_int)),
+                  self.d_pos.cptr, self.d_vel.cptr)
         _lib.call("pvl_surface_center", self.mesh._device(), self.d_pos.cptr, self.d_cp.cptr)
         _lib.call("pvl_surface_center", self.mesh._device(), self.d_vel.cptr, self.d_cpv.cptr)
         _lib.call("pvl_surface_normal", self.mesh._device(), self.d_pos.cptr, self.d_nrm.cptr)
@@ -195,8 +113,8 @@ class UnsteadySolver:
         """Kinematics, rhs, system matrix and wake corrections of one step."""
         self._kinematics(t)
         n = self.mesh.n_surfaces
-        _lib.call("pvl_rhs_normal_velocity", C.c_uint(n), self.d_nrm.cptr, self.d_cpv.cptr,
-                  self.vinf.ctypes.data_as(C.POINTER(C.c_double)), self.d_b.cptr)
+        _lib.call("pvl_rhs_flow", C.c_uint(n), self.d_nrm.cptr, self.d_cpv.cptr, self.d_cp.cptr,
+                  self.flow_desc.ctypes.data_as(C.POINTER(C.c_double)), self.d_b.cptr)
         self.mesh.induction_matrix3(self.tol, self.d_pos, self.d_cp, self.d_nrm, self.d_A, precision=self.prec)
         if self.d_b0 is not None:
             _lib.call("pvl_memcpy_d2d", self.d_b0.cptr, self.d_b.cptr, C.c_size_t(8 * n))
@@ -261,6 +179,218 @@ class UnsteadySolver:
             if callback is not None:
                 callback(self)
         return walls
+
+    # -- geometry, state snapshot and resume ------------------------------------------
+    @property
+    def geometry(self) -> "SimulationGeometry":
+        if getattr(self, "_geometry", None) is None:
+            self._geometry = SimulationGeometry(self.mesh, self.p0, self.frame, closed=self.closed)
+        return self._geometry
+
+    def snapshot(self) -> "SolverState":
+        """The state after the last step (python/pyvl/solver.py:42-108: positions, normals,
+        control points and their velocities, circulation, iteration, wake), copied to host
+        memory; the device state is untouched."""
+        st = SolverState(self.geometry, self.settings(), None)
+        st.positions = self.d_pos.numpy()
+        st.normals = self.d_nrm.numpy()
+        st.control_points = self.d_cp.numpy()
+        st.cp_velocity = self.d_cpv.numpy()
+        st.circulation = self.d_x.numpy()
+        st.iteration = self.iteration
+        st.wake_data = self.wake.save() if self.wake is not None else None
+        return st
+
+    def restore(self, state: "SolverState") -> None:
+        """Resume from a snapshot (no reference counterpart: the reference writes states but
+        has no entry point that continues from one).  The next step is iteration
+        ``state.iteration``: every later step is the one an uninterrupted run makes, bit for
+        bit, since a step depends only on its time, the wake state and the geometry."""
+        n = self.mesh.n_surfaces
+        if state.circulation.shape != (n,):
+            raise ValueError("state does not match this solver's geometry")
+        self.iteration = int(state.iteration)
+        self.d_pos.copy_from(np.ascontiguousarray(state.positions, np.float64))
+        self.d_nrm.copy_from(np.ascontiguousarray(state.normals, np.float64))
+        self.d_cp.copy_from(np.ascontiguousarray(state.control_points, np.float64))
+        self.d_cpv.copy_from(np.ascontiguousarray(state.cp_velocity, np.float64))
+        self.d_x.copy_from(np.ascontiguousarray(state.circulation, np.float64))
+        if self.wake is not None:
+            if state.wake_data is None:
+                raise ValueError("state has no wake but this solver has one")
+            self.wake.set_state(state.wake_data)
+
+    def settings(self) -> SolverSettings:
+        return SolverSettings(self.flow, ModelSettings(self.tol), TimeSettings(1, self.dt, None))
+
+
+class SimulationGeometry:
+    """One body's mesh, initial positions and reference frame: the single-body form of
+    ``SimulationGeometry`` (python/pyvl/geometry.py:389-570) that the post-processing and
+    the solver state read (``mesh``, ``dual``, counts, ``positions_at_time``,
+    ``velocity_at_time``, save/load)."""
+
+    def __init__(self, mesh: Mesh, positions, frame, closed: bool | None = None):
+        self.mesh = mesh
+        self.dual = mesh.compute_dual()
+        self.positions = np.ascontiguousarray(positions, np.float64)
+        self.rf = frame
+        self.closed = is_closed(mesh) if closed is None else bool(closed)
+        self.n_points, self.n_lines, self.n_surfaces = mesh.n_points, mesh.n_lines, mesh.n_surfaces
+
+    def positions_at_time(self, t: float) -> np.ndarray:
+        """python/pyvl/geometry.py:518-539: the evolved frame's to_global_with_offset."""
+        return self.rf.at_time(float(t)).to_global_with_offset(self.positions)
+
+    def velocity_at_time(self, t: float) -> np.ndarray:
+        """python/pyvl/geometry.py:541-570: add_velocity at the current positions, then both
+        taken to the parent, frame by frame."""
+        rf = self.rf.at_time(float(t))
+        pos = np.array(self.positions)
+        v = np.zeros_like(pos)
+        while rf is not None:
+            rf.add_velocity(pos, v)
+            rf.to_parent_with_offset(pos, pos)
+            rf.to_parent_without_offset(v, v)
+            rf = rf.parent
+        return v
+
+    def save(self):
+        from .fio import HirearchicalMap
+        from .reference_frames import rf_to_serial
+
+        out = HirearchicalMap()
+        mesh = HirearchicalMap()
+        mesh.insert_int("n_points", self.mesh.n_points)
+        counts, flat = self.mesh.to_element_connectivity()
+        mesh.insert_array("n_per_element", counts)
+        mesh.insert_array("flattened_elements", flat)
+        out.insert_hirearchycal_map("mesh", mesh)
+        out.insert_array("positions", self.positions)
+        out.insert_hirearchycal_map("reference_frame", rf_to_serial(self.rf))
+        out.insert_int("closed", int(self.closed))
+        return out
+
+    @classmethod
+    def load(cls, group) -> "SimulationGeometry":
+        from .reference_frames import rf_from_serial
+
+        m = group.get_hirearchical_map("mesh")
+        counts = m.get_array("n_per_element")
+        flat = np.asarray(m.get_array("flattened_elements"), np.uint32)
+        offs = np.pad(np.cumsum(counts), (1, 0))
+        mesh = Mesh(m.get_int("n_points"), [flat[offs[i]:offs[i + 1]] for i in range(counts.size)])
+        return cls(mesh, group.get_array("positions"), rf_from_serial(group.get_hirearchical_map("reference_frame")),
+                   closed=bool(group.get_int("closed")))
+
+
+class SolverState:
+    """python/pyvl/solver.py:42-108: the step's arrays, the iteration, the settings, the
+    geometry and the wake's saved state."""
+
+    def __init__(self, geometry: SimulationGeometry, settings: SolverSettings, wake_data: dict | None):
+        n, p = geometry.n_surfaces, geometry.n_points
+        self.geometry = geometry
+        self.settings = settings
+        self.positions = np.empty((p, 3))
+        self.normals = np.empty((n, 3))
+        self.control_points = np.empty((n, 3))
+        self.cp_velocity = np.empty((n, 3))
+        self.circulation = np.empty((n,))
+        self.iteration = 0
+        self.wake_data = wake_data
+
+    def save(self):
+        from .fio import HirearchicalMap
+
+        out = HirearchicalMap()
+        for k in ("positions", "normals", "control_points", "cp_velocity", "circulation"):
+            out.insert_array(k, getattr(self, k))
+        out.insert_int("iteration", int(self.iteration))
+        out.insert_hirearchycal_map("solver_settings", HirearchicalMap.from_dict(self.settings.save()))
+        out.insert_hirearchycal_map("simulation_geometry", self.geometry.save())
+        if self.wake_data is not None:
+            wm = HirearchicalMap()
+            wm.insert_string("type", "WakeModelLineExplicitUnsteady")
+            wm.insert_hirearchycal_map("data", HirearchicalMap.from_dict(self.wake_data))
+            out.insert_hirearchycal_map("wake_model", wm)
+        return out
+
+    @classmethod
+    def load(cls, hmap) -> "SolverState":
+        geometry = SimulationGeometry.load(hmap.get_hirearchical_map("simulation_geometry"))
+        settings = SolverSettings.load(hmap.get_hirearchical_map("solver_settings"))
+        wake = None
+        if "wake_model" in hmap:
+            wake = dict(hmap.get_hirearchical_map("wake_model").get_hirearchical_map("data").to_dict())
+        st = cls(geometry, settings, wake)
+        st.iteration = hmap.get_int("iteration")
+        for k in ("positions", "normals", "control_points", "cp_velocity", "circulation"):
+            setattr(st, k, hmap.get_array(k))
+        return st
+
+
+class SolverResults:
+    """python/pyvl/solver.py:20-39: circulations at the output times and the wake's state at
+    each (host snapshots evaluated on one shared device wake)."""
+
+    def __init__(self, geometry: SimulationGeometry, settings: SolverSettings):
+        self.geometry = geometry
+        self.settings = settings
+        self.wake_models: list = []
+        self.circulations = np.empty((settings.time_settings.output_times.size, geometry.n_surfaces), np.float64)
+
+
+@dataclass(frozen=True)
+class OutputSettings:
+    """File output of the state at each output step (python/pyvl/solver.py:114-142): "JSON"
+    (the reference's JSON layout) or "NPZ" (exact binary; in place of the reference's HDF5)."""
+
+    ftype: str
+    naming_callback: object
+
+    def serialization_fn(self, hmap, path) -> None:
+        from . import fio
+
+        if self.ftype == "JSON":
+            fio.serialize_json(hmap, path)
+        elif self.ftype == "NPZ":
+            fio.serialize_npz(hmap, path)
+        else:
+            raise ValueError(f"The file type ftype={self.ftype!r} is not valid.")
+
+
+def run_solver(solver: UnsteadySolver, time_settings: TimeSettings,
+               output_settings: OutputSettings | None = None) -> SolverResults:
+    """run_solver (python/pyvl/solver.py:145-265) over ``time_settings.nt`` steps of the
+    resident solver, from its current iteration: circulations and wake snapshots at the
+    output steps (every ``output_interval``), the state written through ``output_settings``."""
+    if float(time_settings.dt) != solver.dt:
+        raise ValueError(f"time step {time_settings.dt} differs from the solver's {solver.dt}")
+    settings = SolverSettings(solver.flow, ModelSettings(solver.tol), time_settings)
+    results = SolverResults(solver.geometry, settings)
+    evaluator = None
+    i_out = 0
+    for iteration in range(time_settings.nt):
+        solver.step()
+        it = solver.iteration - 1
+        if time_settings.output_interval is None or time_settings.output_interval == 0 or \
+                iteration % time_settings.output_interval == 0:
+            results.circulations[i_out, :] = solver.d_x.numpy()
+            if solver.wake is not None:
+                if evaluator is None:
+                    w = solver.wake
+                    evaluator = WakeModelLineExplicitUnsteady(w.bordering_nodes, w.shedding_lines,
+                                                              w.adjacent_surfaces, w.vortex_tol, 0.0, w.line_rows,
+                                                              mode=w.mode, precision=w.precision)
+                results.wake_models.append(WakeSnapshot(solver.wake.save(), evaluator))
+            else:
+                results.wake_models.append(None)
+            if output_settings is not None:
+                output_settings.serialization_fn(solver.snapshot().save(),
+                                                 output_settings.naming_callback(it, it * solver.dt))
+            i_out += 1
+    return results
 
 
 def make_wing_wake_solver(n_span: int, n_chord: int, span: float, chord: float, frame, flow, line_rows: int,

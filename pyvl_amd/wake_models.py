@@ -7,9 +7,10 @@ and out, as the reference's in-place numpy updates) or :class:`DeviceArray` (res
 
 ``mode="reference"`` reproduces the reference's convection (free stream + bound-mesh
 induction, :121-128).  ``mode="free"`` also adds the wake's own induction (its
-``get_velocity``, :159-175) to the convection velocity: a free wake.  Only uniform free
-streams are supported (``FlowConditionsUniform``; the velocity is taken from
-``flow.get_velocity``).
+``get_velocity``, :159-175) to the convection velocity: a free wake.  The free stream is
+evaluated per node on the GPU for the fields of :mod:`pyvl_amd.flow_conditions` (uniform and
+rotating); any other object with ``get_velocity`` is taken as uniform, its velocity read at
+the origin.
 """
 
 from __future__ import annotations
@@ -25,9 +26,11 @@ from .device import DeviceArray
 _MODES = {"reference": 0, "free": 1}
 
 
-def _uniform_velocity(flow, time: float) -> np.ndarray:
+def _flow_descriptor(flow, time: float) -> np.ndarray:
+    if hasattr(flow, "device_descriptor"):
+        return np.ascontiguousarray(flow.device_descriptor(), np.float64)
     v = np.asarray(flow.get_velocity(time, np.zeros((1, 3))), np.float64).reshape(-1, 3)[0]
-    return np.ascontiguousarray(v)
+    return np.array([0.0, v[0], v[1], v[2], 0.0, 0.0, 0.0], np.float64)
 
 
 class WakeModelLineExplicitUnsteady:
@@ -112,9 +115,9 @@ class WakeModelLineExplicitUnsteady:
             np.ascontiguousarray(positions, np.float64))
         circ = circulation if isinstance(circulation, DeviceArray) else DeviceArray.from_numpy(
             np.ascontiguousarray(circulation, np.float64))
-        v = _uniform_velocity(flow, time)
-        _lib.call("pvl_wake_update", self._h, C.c_double(float(time)), mesh._device(), pos.cptr, circ.cptr,
-                  v.ctypes.data_as(C.POINTER(C.c_double)), C.c_int(_MODES[self.mode]), C.c_int(self.precision))
+        desc = _flow_descriptor(flow, time)
+        _lib.call("pvl_wake_update_flow", self._h, C.c_double(float(time)), mesh._device(), pos.cptr, circ.cptr,
+                  desc.ctypes.data_as(C.POINTER(C.c_double)), C.c_int(_MODES[self.mode]), C.c_int(self.precision))
 
     def begin_update(self) -> None:
         """Start the next update's wake self-induction on a second stream (free wake only; no
@@ -149,6 +152,28 @@ class WakeModelLineExplicitUnsteady:
         if not isinstance(rhs_in, DeviceArray):
             b.numpy(rhs_in)
 
+    def correct_forces(self, line_forces, geometry, positions, circulation, flow) -> None:
+        """Subtract the first wake row's force from the shedding lines' forces, in place
+        (python/pyvl/wake_models.py:293-330): the row-0 rings as a one-row ring mesh with
+        reduced circulation circulation[:, 0] / (2 pi), in the free stream plus the bound
+        mesh's and the wake's induction at their nodes; shedding line i loses force 4 i + 1
+        of that mesh (the reference's index).  The induction sums run on the GPU."""
+        n = self.shedding_lines.size
+        if getattr(self, "_row_mesh", None) is None:
+            self._row_mesh = Mesh.wake_rings(n, 1)
+            self._row_dual = self._row_mesh.compute_dual()
+        wp, wc = self.state()
+        cpts = np.ascontiguousarray(wp[:, :, 0:2, :].reshape(-1, 3))
+        circ = np.ascontiguousarray(wc[:, 0] / (2 * np.pi))
+        freestream = flow.get_velocity(self.current_time, cpts)
+        mesh = geometry.mesh if hasattr(geometry, "mesh") else geometry
+        ind_v = mesh.point_velocity(self.vortex_tol, np.ascontiguousarray(positions, np.float64),
+                                    np.ascontiguousarray(circulation, np.float64), cpts)
+        self_v = self.get_velocity(cpts)
+        forces = Mesh.line_forces(self._row_mesh, self._row_dual, circ, cpts, freestream + ind_v + self_v)
+        for i, i_line in enumerate(self.shedding_lines):
+            line_forces[i_line, :] -= forces[4 * i + 1, :]
+
     # -- serialisation (python/pyvl/wake_models.py:233-291) ------------------------------
     def save(self) -> dict:
         pos, circ = self.state()
@@ -167,12 +192,19 @@ class WakeModelLineExplicitUnsteady:
     @classmethod
     def load(cls, group: dict, **kw) -> "WakeModelLineExplicitUnsteady":
         out = cls(group["bordering_nodes"], group["shedding_lines"], group["adjacent_surfaces"],
-                  group["vortex_tol"], group["current_time"], group["line_rows"], **kw)
+                  float(group["vortex_tol"]), float(group["current_time"]), int(group["line_rows"]), **kw)
+        out.set_state(group)
+        return out
+
+    def set_state(self, group) -> None:
+        """Load positions, circulation, step count and time from a saved map (same topology)."""
         pos = np.ascontiguousarray(group["wake_positions"], np.float64)
         circ = np.ascontiguousarray(group["circulation"], np.float64)
-        _lib.call("pvl_wake_set_state", out._h, pos.ctypes.data_as(C.c_void_p), circ.ctypes.data_as(C.c_void_p),
+        n = self.shedding_lines.size
+        if pos.shape != (n, 2, self.line_rows + 1, 3) or circ.shape != (n, self.line_rows):
+            raise ValueError("saved wake state does not match this wake's shape")
+        _lib.call("pvl_wake_set_state", self._h, pos.ctypes.data_as(C.c_void_p), circ.ctypes.data_as(C.c_void_p),
                   C.c_uint(int(group["step_count"])), C.c_double(float(group["current_time"])))
-        return out
 
     def __del__(self):
         try:
@@ -181,6 +213,38 @@ class WakeModelLineExplicitUnsteady:
                 self._h = None
         except Exception:
             pass
+
+
+class WakeSnapshot:
+    """A saved wake state (what the reference keeps per output step as
+    ``type(wm).load(wm.save())``, python/pyvl/solver.py:253-254), held in host memory and
+    evaluated on one shared device wake that is loaded with the state on use."""
+
+    def __init__(self, saved: dict, evaluator: "WakeModelLineExplicitUnsteady"):
+        self._saved = saved
+        self._eval = evaluator
+
+    def save(self) -> dict:
+        return self._saved
+
+    def __getattr__(self, name):
+        if name in ("wake_positions", "circulation", "current_time", "step_count", "line_rows", "vortex_tol",
+                    "shedding_lines", "bordering_nodes", "adjacent_surfaces"):
+            return self._saved[name]
+        raise AttributeError(name)
+
+    def _loaded(self) -> "WakeModelLineExplicitUnsteady":
+        ev = self._eval
+        if getattr(ev, "_loaded_from", None) is not self:
+            ev.set_state(self._saved)
+            ev._loaded_from = self
+        return ev
+
+    def get_velocity(self, positions):
+        return self._loaded().get_velocity(positions)
+
+    def correct_forces(self, line_forces, geometry, positions, circulation, flow) -> None:
+        self._loaded().correct_forces(line_forces, geometry, positions, circulation, flow)
 
 
 def line_adjacency_information(mesh: Mesh, dual: Mesh, lines):

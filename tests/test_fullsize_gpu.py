@@ -142,3 +142,45 @@ def test_c3_full_wake_step_matches_oracle(mode):
     assert float(np.max(np.abs(wp - ow.wake_positions)) / np.max(np.abs(ow.wake_positions))) < 1e-10
     assert float(np.max(np.abs(wc - ow.circulation)) / np.max(np.abs(ow.circulation))) < 1e-10
     assert sol.wake.step_count == ow.step_count
+
+
+def test_c5_full_run_gmres_matches_fp64_lu_and_oracle():
+    """The learned-start workload at its full size: tapered wing 64 x 32 = 2,048 panels,
+    pitch + heave (kinematics seed 7), free wake of 300 rows, 300 steps.  GMRES-IR from the
+    minimal-residual projection start (the default iterative start) against the FP64 LU run
+    at every step (<= 1e-10, no LU fallback), and at two sampled steps both against the
+    oracle's FP64 solve (scipy LU) of the step's system, assembled on the CPU from the wake
+    state the GPU run had at that step (<= 1e-10)."""
+    from pyvl_amd import learned as L
+    from pyvl_amd.solver import make_c5_solver
+    from oracle import frames_oracle as FO
+
+    steps, rows = 300, 300
+    judged = {}
+
+    def capture(sol, it):
+        if it in (150, steps - 1):
+            wp, wc = sol.wake.state()
+            judged[it] = (sol.d_pos.numpy(), wp, wc)
+
+    ref = L.run_c5(7, steps, solve="lu64", rows=rows)
+    run = L.run_c5(7, steps, solve="gmres", warm="proj", rows=rows, pre_solve=capture)
+    assert run["fallbacks"] == 0
+    err = np.max(np.abs(run["gamma"] - ref["gamma"]), axis=1) / np.max(np.abs(ref["gamma"]), axis=1)
+    assert np.max(err) < 1e-10
+    sol = make_c5_solver(7, rows=rows)
+    pos0, elements, _ = G.flat_wing(64, 32, 8.0, 1.0, taper=0.5)
+    topo = O.mesh_from_elements(pos0.shape[0], elements)
+    w = sol.wake
+    for it, (pos, wp, wc) in judged.items():
+        cps, nrm = O.surface_center(topo, pos), O.surface_normal(topo, pos)
+        vel = FO.kinematics(sol.frame, sol.p0, it * sol.dt)[1]
+        b = np.vecdot(nrm, -(np.array([1.0, 0.0, 0.0]) - O.surface_center(topo, vel)), axis=1)
+        a = O.induction_matrix3(topo, pos, cps, nrm, 1e-6)
+        ow = WO.OracleWake(w.bordering_nodes, w.shedding_lines, w.adjacent_surfaces, 1e-6, 0.0, rows)
+        ow.wake_positions[...] = wp
+        ow.circulation[...] = wc
+        ow.apply_corrections(cps, nrm, a, b)
+        g_or = sla.lu_solve(sla.lu_factor(a), b)
+        for g in (run["gamma"][it], ref["gamma"][it]):
+            assert float(np.linalg.norm(g - g_or) / np.linalg.norm(g_or)) < 1e-10, it

@@ -12,6 +12,7 @@ import ctypes as C
 import numpy as np
 import pytest
 
+from oracle import frames_oracle as FO
 from oracle import mlp_oracle as MO
 from pyvl_amd import DeviceArray, _lib
 from pyvl_amd import geometry as G
@@ -236,7 +237,7 @@ def test_learned_warm_start_gains():
     w = sol.wake
     for it, (pos, wp, wc) in judged.items():
         cps, nrm = O.surface_center(topo, pos), O.surface_normal(topo, pos)
-        vel = sol.frame.velocities(sol.p0, it * sol.dt)
+        vel = FO.kinematics(sol.frame, sol.p0, it * sol.dt)[1]
         b = np.vecdot(nrm, -(np.array([1.0, 0.0, 0.0]) - O.surface_center(topo, vel)), axis=1)
         a = O.induction_matrix3(topo, pos, cps, nrm, 1e-6)
         ow = OracleWake(w.bordering_nodes, w.shedding_lines, w.adjacent_surfaces, 1e-6, 0.0, rows)
