@@ -152,6 +152,20 @@ int pvl_line_induction_matrix(const pvl_mesh *mesh, double tol, const double *po
  * accumulate != 0 adds into out instead of overwriting. */
 int pvl_point_velocity(const pvl_mesh *mesh, double tol, const double *positions_dev, const double *circulation_dev,
                        unsigned n_pts, const double *pts_dev, double *out_dev, int accumulate, int precision);
+/* No counterpart in the reference.  pvl_point_velocity for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc): the same kernels and chunking, so bit-identical results, but its working
+ * buffers (line geometry and line circulations, 88 B per line, plus per-chunk partial sums,
+ * 24 B per point and line chunk) live in one fixed array of PVL_NOALLOC_SCRATCH_BYTES per
+ * GPU, which the library holds until the process ends; a call that needs more fails with
+ * PVL_ERR_ARG before any launch.  pvl_init must have been called (it looks up that array's
+ * device address).  Every call on a GPU shares the array: calls whose device work overlaps
+ * (e.g. captured replays on other streams) race.  Allocates nothing while the event
+ * profiler (pvl_prof_enable) is off. */
+#define PVL_NOALLOC_SCRATCH_BYTES ((size_t)32 << 20)
+int pvl_point_velocity_noalloc(const pvl_mesh *mesh, double tol, const double *positions_dev,
+                               const double *circulation_dev, unsigned n_pts, const double *pts_dev, double *out_dev,
+                               int accumulate, int precision);
 /* Mesh.surface_normal / Mesh.surface_average_vec3 (src/meshobject.c:956-1062 =
  * src/core/mesh.c:48-68 and :24-46). */
 int pvl_surface_normal(const pvl_mesh *mesh, const double *positions_dev, double *out_dev);

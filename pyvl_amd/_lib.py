@@ -78,6 +78,10 @@ SIGNATURES: dict[str, tuple] = {
         C.c_int,
         [C.c_void_p, C.c_double, _dp, _dp, C.c_uint, _dp, _dp, C.c_int, C.c_int],
     ),
+    "pvl_point_velocity_noalloc": (
+        C.c_int,
+        [C.c_void_p, C.c_double, _dp, _dp, C.c_uint, _dp, _dp, C.c_int, C.c_int],
+    ),
     "pvl_line_forces": (C.c_int, [C.c_void_p, _dp, C.c_uint, C.c_uint, C.c_uint, _dp, _dp, _dp, _dp]),
     "pvl_surface_normal": (C.c_int, [C.c_void_p, _dp, _dp]),
     "pvl_surface_center": (C.c_int, [C.c_void_p, _dp, _dp]),

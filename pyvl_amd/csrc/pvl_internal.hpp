@@ -53,6 +53,11 @@ int side_groups(int per_cu = 5); // workgroup cap for grid-stride kernels on the
 // the side stream's (priority 0), which shares their SIMDs.
 template <int LEVEL = 1> __device__ __forceinline__ void main_stream_priority() { __builtin_amdgcn_s_setprio(LEVEL); }
 int ensure_init();
+// The device the library is bound to (pvl_init), or -1 before initialisation.
+int initialised_device();
+// Set-up of the no-allocation entry point on `device` (current): looks up and keeps the
+// device address of its fixed scratch array.  Called once per device by initialisation.
+int noalloc_setup(int device);
 
 // Kernel timing hook: when profiling is on, brackets the launches made on stream s (default:
 // the library stream) during the scope's lifetime with HIP events and accumulates the

@@ -64,8 +64,17 @@ static int init_locked(int device)
     PVL_HIP(hipSetDevice(device));
     if (g_stream == nullptr)
         PVL_HIP(hipStreamCreateWithFlags(&g_stream, hipStreamNonBlocking));
+    // Failure is not fatal to the rest of the library: the no-allocation entry point repeats
+    // the set-up on its first call and reports the error itself.
+    (void)noalloc_setup(device);
     g_device = device;
     return PVL_OK;
+}
+
+int initialised_device()
+{
+    std::lock_guard<std::mutex> lk(g_mutex);
+    return g_device;
 }
 
 int ensure_init()
