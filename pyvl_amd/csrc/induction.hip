@@ -518,11 +518,11 @@ __global__ void __launch_bounds__(256) chain_segment_kernel(unsigned n_seg, cons
 
 // PPL points per lane; the workgroup's four waves take the pieces of the item's chunk in
 // turn; per-wave sums are combined in LDS in wave order and written per chunk (no atomics).
-// Along a piece, the end node's dr = p - r and |dr|^2 become the next segment's start terms
-// (per pair 8 of the ~96 instructions of line_k_acc are saved).  The segment loop is unrolled
-// by two so that the carried terms are renamed, not copied; 76 VGPRs (capping at 64 spills):
-// four workgroups per CU on the side stream leave each SIMD 208 VGPRs for the library
-// stream's kernels.
+// Along a piece, the end node's dr = p - r becomes the next segment's dr1 (the reference form
+// also carries |dr|^2), so each segment record is one 64-byte scalar load and the accurate
+// forms spend ~66 VALU instructions per pair (cross_m).  The segment loop is unrolled by two
+// so that the carried terms are renamed, not copied; 70 VGPRs: four workgroups per CU on the
+// side stream leave each SIMD 208 or more VGPRs for the library stream's kernels.
 template <int PREC, int PPL>
 __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6)))
     chain_velocity_kernel(unsigned n_pts, const double *__restrict__ pts, const double *__restrict__ nodes,
@@ -559,12 +559,12 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6))
             }
             const double sx = nodes[3 * pc.start_node + 0], sy = nodes[3 * pc.start_node + 1],
                          sz = nodes[3 * pc.start_node + 2];
-            double ax[PPL], ay[PPL], az[PPL], aq[PPL];
+            double ax[PPL], ay[PPL], az[PPL], aq[PPL]; // aq (|p - node|^2) only for the reference form
 #pragma unroll
             for (int i = 0; i < PPL; ++i)
             {
                 ax[i] = px[i] - sx, ay[i] = py[i] - sy, az[i] = pz[i] - sz;
-                aq[i] = ax[i] * ax[i] + ay[i] * ay[i] + az[i] * az[i];
+                aq[i] = PREC == PVL_FP64_REF_FORM ? ax[i] * ax[i] + ay[i] * ay[i] + az[i] * az[i] : 0.0;
             }
             const SegRec *sg = segs + pc.first_seg;
 #pragma unroll 2
@@ -575,27 +575,31 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6))
                 for (int i = 0; i < PPL; ++i)
                 {
                     const double bx = px[i] - r.nx, by = py[i] - r.ny, bz = pz[i] - r.nz;
-                    const double bq = bx * bx + by * by + bz * bz;
                     bool cut;
-                    const double m =
-                        pair_m<PREC>(r.ex, r.ey, r.ez, r.len, ax[i], ay[i], az[i], aq[i], bx, by, bz, bq, tol, tol2, cut);
                     if constexpr (PREC == PVL_FP64_REF_FORM)
                     {
+                        const double bq = bx * bx + by * by + bz * bz;
+                        const double m =
+                            pair_m<PREC>(r.ex, r.ey, r.ez, r.len, ax[i], ay[i], az[i], aq[i], bx, by, bz, bq, tol, tol2, cut);
                         const double kx = cut ? 0.0 : (ay[i] * r.ez - az[i] * r.ey) * m;
                         const double ky = cut ? 0.0 : (az[i] * r.ex - ax[i] * r.ez) * m;
                         const double kz = cut ? 0.0 : (ax[i] * r.ey - ay[i] * r.ex) * m;
                         vx[i] = vx[i] + r.gamma * kx;
                         vy[i] = vy[i] + r.gamma * ky;
                         vz[i] = vz[i] + r.gamma * kz;
+                        aq[i] = bq;
                     }
                     else
                     {
+                        double cx, cy, cz;
+                        const double m =
+                            cross_m<PREC>(r.ex, r.ey, r.ez, r.len, ax[i], ay[i], az[i], tol2, cx, cy, cz, cut);
                         const double wm = cut ? 0.0 : r.gamma * m;
-                        vx[i] = fma(wm, fma(ay[i], r.ez, -(az[i] * r.ey)), vx[i]);
-                        vy[i] = fma(wm, fma(az[i], r.ex, -(ax[i] * r.ez)), vy[i]);
-                        vz[i] = fma(wm, fma(ax[i], r.ey, -(ay[i] * r.ex)), vz[i]);
+                        vx[i] = fma(wm, cx, vx[i]);
+                        vy[i] = fma(wm, cy, vy[i]);
+                        vz[i] = fma(wm, cz, vz[i]);
                     }
-                    ax[i] = bx, ay[i] = by, az[i] = bz, aq[i] = bq;
+                    ax[i] = bx, ay[i] = by, az[i] = bz;
                 }
             }
         }

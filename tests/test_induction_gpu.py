@@ -157,7 +157,7 @@ def test_point_velocity_parity(name, pos, el, prec):
     # rounding, so the difference is a few ulp of the largest velocity
     assert rel_max(got, want) <= 1e-12
     assert rel_fro(got, want) <= 1e-12
-    acc = m.point_velocity(TOL, pos, circ, pts, np.ones_like(pts), accumulate=True)
+    acc = m.point_velocity(TOL, pos, circ, pts, np.ones_like(pts), accumulate=True, precision=prec)
     np.testing.assert_allclose(acc, got + 1.0, rtol=0, atol=1e-14 * (1 + np.max(np.abs(got))))
 
 
