@@ -41,7 +41,7 @@ PEAK_FP32_TFLOPS = 157.3  # MI355X FP32 (vector and MFMA)
 # WRITE_SIZE in two runs of this script (--copy-calibration), each scaled by what the same
 # counter reads for the script's own 1 GiB copy (FETCH x 2.0, WRITE x 1.0); the commands and
 # numbers are in profiles/wake_self_velocity_c3_fp64_traffic.md.
-SELF_VELOCITY_HBM_BYTES = 61.14e6
+SELF_VELOCITY_HBM_BYTES = 71.82e6
 PEAK_HBM_GBS = 8000.0
 FLOP_PER_PAIR = 87.0  # induction pair (src/core/flow_solver.c:114-135, atan2 counted as 20)
 FLOP_WEIGHT = 6.0  # Gamma weighting of a point-velocity sum
@@ -124,7 +124,7 @@ def roofline_from(prof) -> dict:
                 f"algorithmic {FLOP_PER_PAIR + FLOP_WEIGHT:.0f} FLOP per (segment, point) pair, "
                 f"{pairs / max(cnt, 1):.4g} pairs per launch, avg {avg_ms:.3f} ms over {cnt} launches "
                 "(HIP events on the side stream the kernel is launched on, around this kernel only, over "
-                "the timed region; it runs beside the bound solve, capped at 4 workgroups per CU)",
+                "the timed region; it runs beside the bound solve, capped at 5 workgroups per CU)",
     }
 
 

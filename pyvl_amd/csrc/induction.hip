@@ -521,10 +521,11 @@ __global__ void __launch_bounds__(256) chain_segment_kernel(unsigned n_seg, cons
 // Along a piece, the end node's dr = p - r becomes the next segment's dr1 (the reference form
 // also carries |dr|^2), so each segment record is one 64-byte scalar load and the accurate
 // forms spend ~66 VALU instructions per pair (cross_m).  The segment loop is unrolled by two
-// so that the carried terms are renamed, not copied; 70 VGPRs: four workgroups per CU on the
-// side stream leave each SIMD 208 or more VGPRs for the library stream's kernels.
+// so that the carried terms are renamed, not copied.  Capped at 64 VGPRs (a few SGPRs spill to
+// VGPR lanes, none to memory): five workgroups per CU on the side stream (five waves per SIMD)
+// leave each SIMD 192 VGPRs, room for two 96-VGPR waves of the library stream's kernels.
 template <int PREC, int PPL>
-__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6)))
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8)))
     chain_velocity_kernel(unsigned n_pts, const double *__restrict__ pts, const double *__restrict__ nodes,
                           const ChainPiece *__restrict__ pieces, unsigned n_pieces, unsigned cr,
                           const SegRec *__restrict__ segs, unsigned pieces_per_chunk, double *__restrict__ partial,

@@ -1080,6 +1080,43 @@ __global__ void __launch_bounds__(NT) trsv_flow_kernel(const T *__restrict__ A, 
         __hip_atomic_store(&flag[t], epoch, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// s4[j] += sum over c in [c0, c1), c = lane mod 64, of arow[j][c] * x[c], for four factor rows:
+// each lane walks its columns in increasing order (the association of a lane-strided dot
+// product); the loads of four columns per row are issued before their FMAs, so that 16 of a
+// lane's loads are in flight at once (the single-workgroup solves are bound by how many bytes
+// one CU keeps in flight, not by arithmetic).
+template <typename T>
+__device__ __forceinline__ void dot4_lane_strided(const T *const (&arow)[4], const T *__restrict__ x, int c0, int c1,
+                                                  int lane, T (&s4)[4])
+{
+    constexpr int U = 4;
+    for (int cb = c0 + lane; cb < c1; cb += 64 * U)
+    {
+        T av[U][4], xv[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+        {
+            const int c = cb + 64 * u;
+            const bool ok = c < c1;
+            const int cc = ok ? c : cb; // cb < c1: an in-range column
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+                av[u][j] = arow[j][cc];
+            xv[u] = ok ? x[cc] : T(0);
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+        {
+            if (cb + 64 * u < c1)
+            {
+#pragma unroll
+                for (int j = 0; j < 4; ++j)
+                    s4[j] = fma(av[u][j], xv[u], s4[j]);
+            }
+        }
+    }
+}
+
 // Whole triangular solve in ONE workgroup (512 threads, at most 96 VGPRs: it fits on a CU
 // beside the side stream's point-velocity waves) for small systems: x (n x nrhs) lives in LDS; blocks of 64
 // rows are taken in solve order, each first reduced by the already solved part (dot products
@@ -1169,6 +1206,128 @@ __global__ void __launch_bounds__(kTrsvWgThreads) __attribute__((amdgpu_waves_pe
     }
     for (int i = tid; i < n * nrhs; i += kTrsvWgThreads)
         B[i] = x[i];
+}
+
+// Single right-hand side, whole triangular solve in ONE workgroup with look-ahead (the time
+// loop's solve).  Blocks of BS rows are solved in order; block s's reduction by the solved
+// part is split into the blocks solved up to s - 2 ("old", a dot product over contiguous
+// factor rows, as in trsv_wg_kernel) and block s - 1 ("new").  While wave 0 solves block s
+// (the new part from the staged off-diagonal block, then substitution with the staged
+// diagonal block, LAPACK dtrsv order), waves 1-7 stage block s + 1's diagonal block and its
+// off-diagonal block against block s, and form its old part: only wave 0's short chain is
+// on the critical path.  Each row is reduced once by old + new (one subtraction per block,
+// as a blocked dtrsv).  BS = 32 keeps the LDS (x and four BS x (BS + 1) tiles: 42 KB for
+// n = 1,024 in FP64) small enough to share a CU with the side stream's workgroups.
+constexpr int kTrsvLaThreads = 512;
+constexpr int kTrsvLaBs = 32;
+constexpr size_t kTrsvLaMaxX = 32 * 1024; // x in LDS: n <= 4,096 (FP64)
+template <typename T, bool LOWER, int BS>
+__global__ void __launch_bounds__(kTrsvLaThreads) __attribute__((amdgpu_waves_per_eu(5)))
+    trsv_la_kernel(const T *__restrict__ A, int lda, int n, T *__restrict__ b)
+{
+    main_stream_priority<3>();
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    T *x = (T *)smem_raw; // n
+    __shared__ T Dt[2][BS][BS + 1], Ot[2][BS][BS + 1], sold[2][BS];
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int nb = (n + BS - 1) / BS;
+    auto blk = [&](int s) { return LOWER ? s : nb - 1 - s; };
+    for (int i = tid; i < n; i += kTrsvLaThreads)
+        x[i] = b[i];
+    // helpers' work for solve step s (block blk(s)), done during step s - 1 (or up front for
+    // s = 0): stage the diagonal block and the off-diagonal block against blk(s - 1), and the
+    // old part over the blocks solved at steps <= s - 2
+    auto prepare = [&](int s, int htid, int hw, int nhw) {
+        const int ib = blk(s), r0 = ib * BS, nr = min(BS, n - r0);
+        const int buf = s & 1;
+        for (int idx = htid; idx < BS * BS; idx += nhw * 64)
+        {
+            const int r = idx / BS, c = idx % BS;
+            Dt[buf][r][c] = (r < nr && c < nr) ? A[(size_t)(r0 + r) * lda + r0 + c] : (r == c ? T(1) : T(0));
+        }
+        if (s >= 1)
+        {
+            const int p0 = blk(s - 1) * BS, np = min(BS, n - p0);
+            for (int idx = htid; idx < BS * BS; idx += nhw * 64)
+            {
+                const int r = idx / BS, c = idx % BS;
+                Ot[buf][r][c] = (r < nr && c < np) ? A[(size_t)(r0 + r) * lda + p0 + c] : T(0);
+            }
+        }
+        // old columns: blocks solved at steps <= s - 2
+        int c0 = 0, c1 = 0;
+        if (s >= 2)
+        {
+            if (LOWER)
+                c0 = 0, c1 = (s - 1) * BS;
+            else
+                c0 = (nb - s + 1) * BS, c1 = n;
+        }
+        for (int g = hw; g < BS / 4; g += nhw)
+        {
+            const int ra = 4 * g;
+            const T *arow[4];
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+                arow[j] = A + (size_t)(r0 + min(ra + j, nr - 1)) * lda;
+            T s4[4] = {T(0), T(0), T(0), T(0)};
+            dot4_lane_strided<T>(arow, x, c0, c1, lane, s4);
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+            {
+                T s2 = s4[j];
+                for (int off = 32; off > 0; off >>= 1)
+                    s2 += __shfl_xor(s2, off);
+                if (lane == 0)
+                    sold[buf][ra + j] = s2;
+            }
+        }
+    };
+    __syncthreads();
+    prepare(0, tid, wv, 8);
+    __syncthreads();
+    for (int s = 0; s < nb; ++s)
+    {
+        const int buf = s & 1;
+        if (wv == 0)
+        {
+            const int ib = blk(s), r0 = ib * BS, nr = min(BS, n - r0);
+            const int r = lane < BS ? lane : BS - 1;
+            T snew = T(0);
+            if (s >= 1)
+            {
+                const int p0 = blk(s - 1) * BS;
+                T a0 = T(0), a1 = T(0);
+#pragma unroll 4
+                for (int k = 0; k < BS; k += 2)
+                {
+                    a0 = fma(Ot[buf][r][k], x[min(p0 + k, n - 1)], a0);
+                    a1 = fma(Ot[buf][r][k + 1], x[min(p0 + k + 1, n - 1)], a1);
+                }
+                snew = a0 + a1;
+            }
+            T v = lane < nr ? x[r0 + lane] - (sold[buf][r] + snew) : T(0);
+            const T dd = Dt[buf][r][r];
+            for (int kk = 0; kk < BS; ++kk)
+            {
+                const int k = LOWER ? kk : BS - 1 - kk;
+                if (!LOWER && lane == k)
+                    v = v / dd;
+                const T vk = bcast_lane(v, k);
+                if (LOWER ? (lane > k && lane < BS) : lane < k)
+                    v = fma(-Dt[buf][r][k], vk, v);
+            }
+            if (lane < nr)
+                x[r0 + lane] = v;
+        }
+        else if (s + 1 < nb)
+        {
+            prepare(s + 1, tid - 64, wv - 1, 7);
+        }
+        __syncthreads();
+    }
+    for (int i = tid; i < n; i += kTrsvLaThreads)
+        b[i] = x[i];
 }
 
 // R = B - A X on FP64 MFMA (v_mfma_f64_16x16x4f64), for the refinement's FP64 residuals: a
@@ -1626,7 +1785,15 @@ template <typename T> int solve_impl(pvl_lu *h, T *B, int nrhs)
         }
         PVL_HIP(hipGetLastError());
         const size_t xbytes = sizeof(T) * (size_t)n * nq;
-        if (xbytes <= (size_t)kTrsvWgBytes)
+        if (nq == 1 && xbytes <= kTrsvLaMaxX)
+        {
+            ProfScope prof(sizeof(T) == 8 ? "lu_trsv_f64" : "lu_trsv_f32");
+            const T *Ap = (const T *)h->lu;
+            trsv_la_kernel<T, true, kTrsvLaBs><<<1, kTrsvLaThreads, xbytes, stream()>>>(Ap, n, n, work);
+            trsv_la_kernel<T, false, kTrsvLaBs><<<1, kTrsvLaThreads, xbytes, stream()>>>(Ap, n, n, work);
+            PVL_HIP(hipGetLastError());
+        }
+        else if (xbytes <= (size_t)kTrsvWgBytes)
         {
             ProfScope prof(sizeof(T) == 8 ? "lu_trsv_f64" : "lu_trsv_f32");
             const T *Ap = (const T *)h->lu;

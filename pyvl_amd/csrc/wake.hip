@@ -611,7 +611,7 @@ int pvl_wake_begin_update(pvl_wake *w, int mode, int precision)
     // wake nodes (pvl_wake_update evaluates at a copy of the same values)
     PVL_HIP(hipEventRecord(w->ev_ready, stream()));
     PVL_HIP(hipStreamWaitEvent(s, w->ev_ready, 0));
-    if (int st = self_partials(w, n_old, w->d_upos, precision, s, (unsigned)side_groups(w->chains ? 4 : 5)))
+    if (int st = self_partials(w, n_old, w->d_upos, precision, s, (unsigned)side_groups(5)))
         return st;
     PVL_HIP(hipEventRecord(w->ev_done, s));
     w->pending = true;

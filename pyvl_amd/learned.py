@@ -215,6 +215,32 @@ class Projector:
         return V @ c
 
 
+class ProjectionStart:
+    """Default start of ``UnsteadySolver(solve="gmres")``: the minimal-residual combination
+    (:class:`Projector`) of the last ``k`` circulations the solver produced (Fischer's
+    projection for successive right-hand sides); zero at the first step.  Called by the
+    solver before each solve, when its ``d_x`` still holds the previous step's circulation.
+    A jump in the iteration count (a restored state) restarts the history from the
+    circulation the solver then holds."""
+
+    def __init__(self, n: int, k: int = 3):
+        self.proj = Projector(n, k_max=k)
+        self.k = k
+        self.hist: list[np.ndarray] = []
+        self.last: int | None = None
+
+    def __call__(self, sol) -> np.ndarray | None:
+        it = int(sol.iteration)
+        if it == 0:
+            self.hist, self.last = [], 0
+            return None
+        if self.last is None or it != self.last + 1:
+            self.hist = []
+        self.last = it
+        self.hist = (self.hist + [sol.d_x.numpy()])[-self.k:]
+        return self.proj.start(sol.d_A, sol.d_b.numpy(), self.hist)
+
+
 class Normaliser:
     """Feature and target standardisation from training statistics."""
 

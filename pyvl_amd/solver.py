@@ -87,10 +87,15 @@ class UnsteadySolver:
         self.iteration = 0
         self.ir_iters: list[int] = []
         # "gmres": FP32 GMRES in FP64 refinement from warm_start(self) (numpy (n,) or None
-        # for zero) to max|b - A x| <= gmres_tol max|b|, LU64 fallback when it does not
-        # converge; gmres_stats: (corrections, GMRES steps, start and final relative
-        # residual, converged) per solve
+        # for zero; by default the minimal-residual projection on the last three
+        # circulations, learned.ProjectionStart) to max|b - A x| <= gmres_tol max|b|, LU64
+        # fallback when it does not converge; gmres_stats: (corrections, GMRES steps, start
+        # and final relative residual, converged) per solve
         self.warm_start = None
+        if solve == "gmres":
+            from .learned import ProjectionStart
+
+            self.warm_start = ProjectionStart(n)
         self.gmres_tol = 1e-12
         self.gmres_stats: list[tuple] = []
         self.fallbacks = 0

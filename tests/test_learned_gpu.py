@@ -152,6 +152,25 @@ def test_gmres_wing_matches_lu():
     assert ok3 and outer3 == 0 and steps3 == 0
 
 
+def test_gmres_time_loop_default_start_is_projection():
+    """UnsteadySolver(solve="gmres") starts each solve from the projection on its last three
+    circulations by default: the same circulations as the FP64 LU loop to 1e-10, and from the
+    third step on a start residual far below the zero start's 1."""
+    from pyvl_amd.solver import FlowConditionsUniform, PitchHeaveFrame, make_wing_wake_solver
+
+    frame = PitchHeaveFrame(np.radians(5.0), 0.05, 0.8, 0.25, 0.3, 1.1)
+    flow = FlowConditionsUniform(1.0, 0.0, 0.0)
+    a = make_wing_wake_solver(16, 4, 4.0, 1.0, frame, flow, 10, 0.0625, mode="free", solve="gmres")
+    b = make_wing_wake_solver(16, 4, 4.0, 1.0, frame, flow, 10, 0.0625, mode="free")
+    assert isinstance(a.warm_start, L.ProjectionStart)
+    for k in range(12):
+        ga, gb = a.step().numpy(), b.step().numpy()
+        assert _rel(ga, gb) < 1e-10, k
+    rel0 = [st[2] for st in a.gmres_stats]
+    assert rel0[0] == 1.0 and max(rel0[3:]) < 1e-3
+    assert a.fallbacks == 0
+
+
 @pytest.mark.parametrize("n", [1, 100, 1000, 1537])
 def test_gmres_random_matches_lu(n):
     rng = np.random.default_rng(n)

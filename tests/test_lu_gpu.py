@@ -85,6 +85,29 @@ def test_block_solve_paths(prec, n, nrhs):
     assert np.array_equal(x2, x)
 
 
+@pytest.mark.parametrize("prec,n", [("fp64", 1), ("fp64", 5), ("fp64", 31), ("fp64", 32), ("fp64", 33),
+                                    ("fp64", 100), ("fp64", 1000), ("fp64", 1024), ("fp64", 2048),
+                                    ("fp64", 4096), ("fp32", 77), ("fp32", 1024), ("fp32", 4096)])
+def test_single_rhs_lookahead_solve(prec, n):
+    """One right-hand side with x in LDS takes the look-ahead solve (32-row blocks; the
+    reduction of block s split into blocks up to s - 2, formed while block s - 1 is being
+    solved, and block s - 1): against a host solve from the same factors (FP64 1e-12; FP32
+    1e-5, its own rounding), and against the multi-right-hand-side solve of the same column
+    (one workgroup, 64-row blocks: another association of the same sums).  Ragged last
+    blocks and single-block systems are covered."""
+    rng = np.random.default_rng(300 + n)
+    a = rng.normal(size=(n, n)) + 2 * n ** 0.5 * np.eye(n)
+    b = rng.normal(size=n)
+    h = LA.LU(n, LA.FP64 if prec == "fp64" else LA.FP32).factor(a)
+    dt = np.float64 if prec == "fp64" else np.float32
+    x = h.solve(b.astype(dt)).astype(np.float64)
+    x_ref = _solve_with_factors(h.factors(), h.pivots(), b)
+    assert _rel(x, x_ref) < (1e-12 if prec == "fp64" else 1e-5)
+    x2 = h.solve(np.stack([b, b], axis=1).astype(dt)).astype(np.float64)
+    assert _rel(x, x2[:, 0]) < (1e-13 if prec == "fp64" else 1e-6)
+    assert np.array_equal(h.solve(b.astype(dt)).astype(np.float64), x)
+
+
 def test_block_solve_after_refactor():
     """One FP32 handle factored twice (different matrices): the second solve uses the second
     factorisation's diagonal-block inverses, not the first's."""
