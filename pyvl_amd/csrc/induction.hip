@@ -61,14 +61,14 @@ __device__ __forceinline__ unsigned wave_id() { return __builtin_amdgcn_readfirs
 __device__ __forceinline__ unsigned lane_id() { return threadIdx.x % kWave; }
 
 // ---------------------------------------------------------------------------------------
-// Surface-reduced assembly: one lane per control point, the wave walks a run of surfaces
-// (wave-uniform, so surface/line metadata and line geometry come through scalar loads).
-// Each surface accumulates +-K in its CSR order (src/core/flow_solver.c:188-200) and, for
-// NORMAL, dots with the control point's normal (:202).  K of the lines of the previous
-// surface is kept in registers: consecutive surfaces of structured meshes share a line.
-// Results are staged in LDS and written row-contiguous.
-//   NORMAL: out[n_cpts][n_surf]      (induction_matrix3)
+// Surface-reduced induction tensor: one lane per control point, the wave walks a run of
+// surfaces (wave-uniform, so surface/line metadata and line geometry come through scalar
+// loads).  Each surface accumulates +-K in its CSR order (src/core/flow_solver.c:141-172).
+// K of the lines of the previous surface is kept in registers: consecutive surfaces of
+// structured meshes share a line.  Results are staged in LDS and written row-contiguous.
 //   VEC:    out[n_cpts][n_surf][3]   (induction_matrix)
+//   NORMAL: out[n_cpts][n_surf], dotted with the control point's normal (:202); the library
+//           assembles induction_matrix3 with assembly_kernel instead.
 template <bool NORMAL, int PREC, int SURF_PER_WAVE>
 __global__ void __launch_bounds__(kBlock)
     surface_induction_kernel(unsigned n_cpts, unsigned n_surf, const uint32_t *__restrict__ offsets,
@@ -174,6 +174,92 @@ __global__ void __launch_bounds__(kBlock)
         double *dst = out + ((size_t)cpr * n_surf + s_tile0) * kComp;
         for (unsigned c = lane; c < row_len; c += kWave)
             dst[c] = tile[r * kRow + c];
+    }
+}
+
+// Normal-reduced assembly (induction_matrix3), out[n_cpts][n_surf]: one lane per control
+// point, one wave per run of kAsmSurfaces consecutive surfaces executing the run's program
+// (build_assembly_program): each line of the run is evaluated once, as n . K, and kept in an
+// LDS slot while a later surface needs it; a surface's value is the sum of its lines' +-n.K
+// in CSR order (src/core/flow_solver.c:188-202 sums the vectors first and then dots with the
+// normal: the same terms, another association).  Program entries are wave-uniform (scalar
+// loads and branches); per lane the inner step is one K and one LDS access at most.  The run's
+// results are staged in LDS and stored as rows of kAsmSurfaces consecutive doubles.
+template <int PREC>
+__global__ void __launch_bounds__(kBlock)
+    assembly_kernel(unsigned n_cpts, unsigned n_surf, const uint32_t *__restrict__ run_off,
+                    const uint32_t *__restrict__ prog, const LineGeom *__restrict__ geom,
+                    const double *__restrict__ cpts, const double *__restrict__ normals, double *__restrict__ out,
+                    double tol)
+{
+    main_stream_priority();
+    __shared__ double slots[4][kAsmSlots][kWave];
+    __shared__ double res[4][kAsmSurfaces][kWave + 1];
+    const unsigned w = wave_id(), lane = lane_id();
+    const unsigned cp0 = blockIdx.y * kCptTile;
+    const unsigned cp = cp0 + lane;
+    const unsigned cpc = cp < n_cpts ? cp : n_cpts - 1;
+    const double px = cpts[3 * cpc + 0], py = cpts[3 * cpc + 1], pz = cpts[3 * cpc + 2];
+    const double nx = normals[3 * cpc + 0], ny = normals[3 * cpc + 1], nz = normals[3 * cpc + 2];
+    const double tol2 = tol * tol;
+    const unsigned run = blockIdx.x * 4 + w;
+    const unsigned s0 = run * kAsmSurfaces;
+    if (s0 >= n_surf)
+        return;
+    const uint32_t k0 = run_off[run], k1 = run_off[run + 1];
+    unsigned sl = 0;
+    double acc = 0.0;
+    // pairs of entries: both entries' scalar loads (program words and line geometry) are
+    // issued before either is evaluated, so one wait covers two evaluations (scalar loads
+    // return out of order: a wave can only wait for all of them)
+    auto step = [&](uint32_t e, uint32_t op, const LineGeom &g) {
+        const uint32_t kind = op & 3u, slot = (op >> 2) & 7u;
+        double v;
+        if (kind == kAsmLoad)
+            v = slots[w][slot][lane];
+        else if (kind == kAsmNone)
+            v = 0.0;
+        else
+        {
+            v = line_nk<PREC>(g, px, py, pz, nx, ny, nz, tol, tol2);
+            if (kind == kAsmEvalStore)
+                slots[w][slot][lane] = v;
+        }
+        acc = (e & kReversed) ? acc - v : acc + v;
+        if (op & kAsmEnd)
+        {
+            res[w][sl][lane] = acc;
+            acc = 0.0;
+            ++sl;
+        }
+    };
+    uint32_t k = k0;
+    for (; k + 1 < k1; k += 2)
+    {
+        // a kAsmNone entry names line 0 (the geometry buffer always has one element)
+        const uint32_t ea = prog[2 * k], opa = prog[2 * k + 1], eb = prog[2 * k + 2], opb = prog[2 * k + 3];
+        const LineGeom ga = geom[ea & kInvalidId], gb = geom[eb & kInvalidId];
+        // both geometries in SGPRs here (the compiler would otherwise sink the second load
+        // into the branch that uses it, after the first evaluation)
+        asm volatile("" ::"s"(ga.r1x), "s"(ga.r2x), "s"(ga.ex), "s"(ga.len), "s"(gb.r1x), "s"(gb.r2x), "s"(gb.ex),
+                     "s"(gb.len));
+        step(ea, opa, ga);
+        step(eb, opb, gb);
+    }
+    if (k < k1)
+    {
+        const uint32_t ea = prog[2 * k], opa = prog[2 * k + 1];
+        const LineGeom ga = geom[ea & kInvalidId];
+        step(ea, opa, ga);
+    }
+    // rows of the run: lane = 8 rows x 8 surfaces per store instruction
+    const unsigned ns = (n_surf - s0) < kAsmSurfaces ? (n_surf - s0) : kAsmSurfaces;
+    const unsigned c = lane % kAsmSurfaces;
+    for (unsigned r = lane / kAsmSurfaces; r < kCptTile; r += kWave / kAsmSurfaces)
+    {
+        const unsigned p = cp0 + r;
+        if (p < n_cpts && c < ns)
+            out[(size_t)p * n_surf + s0 + c] = res[w][c][r];
     }
 }
 
@@ -470,7 +556,7 @@ int launch_surface(const pvl_mesh *m, double tol, const double *pos, unsigned n_
     LineGeom *g;
     if (int st = compute_line_geom(m, pos, tol, &g))
         return st;
-    constexpr int spw = NORMAL ? 16 : 8;
+    constexpr int spw = NORMAL ? kAsmSurfaces : 8;
     const dim3 grid((m->n_surfaces + 4 * spw - 1) / (4 * spw), (n_cpts + kCptTile - 1) / kCptTile);
     if (grid.y > 65535u)
     {
@@ -479,8 +565,12 @@ int launch_surface(const pvl_mesh *m, double tol, const double *pos, unsigned n_
     }
     ProfScope prof(NORMAL ? "induction_matrix3" : "induction_matrix", (double)n_cpts * (double)m->n_lines);
     dispatch_precision(precision, [&](auto P) {
-        surface_induction_kernel<NORMAL, decltype(P)::value, spw><<<grid, kBlock, 0, stream()>>>(
-            n_cpts, m->n_surfaces, m->d_offsets, m->d_entries, g, cpts, normals, out, tol);
+        if constexpr (NORMAL)
+            assembly_kernel<decltype(P)::value><<<grid, kBlock, 0, stream()>>>(
+                n_cpts, m->n_surfaces, m->d_asm_off, m->d_asm, g, cpts, normals, out, tol);
+        else
+            surface_induction_kernel<NORMAL, decltype(P)::value, spw><<<grid, kBlock, 0, stream()>>>(
+                n_cpts, m->n_surfaces, m->d_offsets, m->d_entries, g, cpts, normals, out, tol);
     });
     PVL_HIP(hipGetLastError());
     return PVL_OK;

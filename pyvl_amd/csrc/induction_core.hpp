@@ -39,11 +39,42 @@ enum
 // pyvl_amd/csrc/tools/fit_atan.py (Lawson iteration in 60-digit arithmetic): P of degree 16
 // for FP64 (maximum relative error of the polynomial 1.1e-15, ~5 ulp: the pair's other
 // rounding errors are of that order), degree 7 for FP32 (1.7e-8).
-__device__ __forceinline__ double atan_poly64(double a)
+// SGPR_COEF: the Horner steps as one inline-asm block of v_fma_f64 with the coefficients in
+// SGPRs.  In kernels whose loop has enough control flow around the evaluation the compiler
+// otherwise keeps the loop-invariant coefficients in VGPRs and emits every step as a copy
+// plus the two-address v_fmac_f64 (twice the instructions); the same 15 FMAs in the same
+// order, so the results are bit-identical.
+template <bool SGPR_COEF = false> __device__ __forceinline__ double atan_poly64(double a)
 {
     const double s = a * a;
     double p = -4.7926633120228785553e-5;
     p = fma(p, s, 4.800660567800612957e-4);
+    if constexpr (SGPR_COEF)
+    {
+        asm("v_fma_f64 %0, %0, %1, %2\n\t"
+            "v_fma_f64 %0, %0, %1, %3\n\t"
+            "v_fma_f64 %0, %0, %1, %4\n\t"
+            "v_fma_f64 %0, %0, %1, %5\n\t"
+            "v_fma_f64 %0, %0, %1, %6\n\t"
+            "v_fma_f64 %0, %0, %1, %7\n\t"
+            "v_fma_f64 %0, %0, %1, %8\n\t"
+            "v_fma_f64 %0, %0, %1, %9\n\t"
+            "v_fma_f64 %0, %0, %1, %10\n\t"
+            "v_fma_f64 %0, %0, %1, %11\n\t"
+            "v_fma_f64 %0, %0, %1, %12\n\t"
+            "v_fma_f64 %0, %0, %1, %13\n\t"
+            "v_fma_f64 %0, %0, %1, %14\n\t"
+            "v_fma_f64 %0, %0, %1, %15\n\t"
+            "v_fma_f64 %0, %0, %1, %16"
+            : "+v"(p)
+            : "v"(s), "s"(-2.2725728333658927666e-3), "s"(6.8183912427924839848e-3),
+              "s"(-1.4739706399322470277e-2), "s"(2.4909137877934616446e-2), "s"(-3.5142229350564300719e-2),
+              "s"(4.3938991765738211806e-2), "s"(-5.1356486201903778769e-2), "s"(5.8483233451193392607e-2),
+              "s"(-6.6598275562737410628e-2), "s"(7.6913027337110721971e-2), "s"(-9.090805442522420044e-2),
+              "s"(1.1111104041852547122e-1), "s"(-1.4285713995249626285e-1), "s"(1.9999999993898812193e-1),
+              "s"(-3.3333333333288106455e-1));
+        return fma(a * s, p, a);
+    }
     p = fma(p, s, -2.2725728333658927666e-3);
     p = fma(p, s, 6.8183912427924839848e-3);
     p = fma(p, s, -1.4739706399322470277e-2);
@@ -96,14 +127,14 @@ __device__ __forceinline__ double pvl_atan2(double y, double x)
 
 // atan2(-ay, x) for ay >= 0 (the segment's angle below: its sine is never positive), as
 // pvl_atan2 without the sign transfer.
-__device__ __forceinline__ double pvl_atan2_neg(double ay, double x)
+template <bool SGPR_COEF = false> __device__ __forceinline__ double pvl_atan2_neg(double ay, double x)
 {
     const double ax = fabs(x);
     const double mn = fmin(ax, ay), mx = fmax(ax, ay);
     double r = __builtin_amdgcn_rcp(mx);
     r = fma(r, fma(-mx, r, 1.0), r);
     const double a = mn * r;
-    double t = atan_poly64(a);
+    double t = atan_poly64<SGPR_COEF>(a);
     t = (ay > ax) ? (1.57079632679489661923 - t) : t;
     t = (x < 0.0) ? (3.14159265358979323846 - t) : t;
     return -t;
@@ -135,7 +166,7 @@ __device__ __forceinline__ double pvl_rsqrt(double x)
 // q2 likewise (each dot product left to right, no FMA, as the reference), e = the unit
 // direction.  The cut-off is decided here.  Split from line_m so that a walk along a chain of
 // segments can reuse one endpoint's terms for the next segment (bit-identical values).
-template <int PREC>
+template <int PREC, bool SGPR_COEF = false>
 __device__ __forceinline__ double pair_m(double ex, double ey, double ez, double len, double d1x, double d1y,
                                          double d1z, double q1, double d2x, double d2y, double d2z, double q2,
                                          double tol, double tol2, bool &cut)
@@ -167,7 +198,7 @@ __device__ __forceinline__ double pair_m(double ex, double ey, double ez, double
         {
             const double ih = pvl_rsqrt(h2);
             const double h = h2 * ih;
-            const double dphi = pvl_atan2_neg(len * h, fma(t1, t2, h2));
+            const double dphi = pvl_atan2_neg<SGPR_COEF>(len * h, fma(t1, t2, h2));
             m = dphi * ih;
         }
         else
@@ -222,7 +253,7 @@ __device__ __forceinline__ double cross_m(double ex, double ey, double ez, doubl
 }
 
 // Scalar factor of K (angle / h; the cut-off is decided here) and dr1 = p - r1.
-template <int PREC>
+template <int PREC, bool SGPR_COEF = false>
 __device__ __forceinline__ double line_m(const LineGeom &g, double px, double py, double pz, double tol, double tol2,
                                          double &d1x, double &d1y, double &d1z, bool &cut)
 {
@@ -230,7 +261,7 @@ __device__ __forceinline__ double line_m(const LineGeom &g, double px, double py
     const double d2x = px - g.r2x, d2y = py - g.r2y, d2z = pz - g.r2z;
     const double q1 = d1x * d1x + d1y * d1y + d1z * d1z;
     const double q2 = d2x * d2x + d2y * d2y + d2z * d2z;
-    return pair_m<PREC>(g.ex, g.ey, g.ez, g.len, d1x, d1y, d1z, q1, d2x, d2y, d2z, q2, tol, tol2, cut);
+    return pair_m<PREC, SGPR_COEF>(g.ex, g.ey, g.ez, g.len, d1x, d1y, d1z, q1, d2x, d2y, d2z, q2, tol, tol2, cut);
 }
 
 // K(line, p).  The reference form keeps the reference's cross product and products; the
@@ -260,6 +291,32 @@ __device__ __forceinline__ void line_k(const LineGeom &g, double px, double py, 
         kx = cut ? 0.0 : cx * m;
         ky = cut ? 0.0 : cy * m;
         kz = cut ? 0.0 : cz * m;
+    }
+}
+
+// n . K(line, p) for the normal-reduced assembly.  The reference form dots the reference's K
+// (reference rounding of every product); the others form m (n . c) with c = dr1 x e and the
+// dot as FMAs, and decide the cut-off once for the scalar.
+template <int PREC>
+__device__ __forceinline__ double line_nk(const LineGeom &g, double px, double py, double pz, double nx, double ny,
+                                          double nz, double tol, double tol2)
+{
+    if constexpr (PREC == PVL_FP64_REF_FORM)
+    {
+        double kx, ky, kz;
+        line_k<PREC>(g, px, py, pz, tol, tol2, kx, ky, kz);
+        return kx * nx + ky * ny + kz * nz;
+    }
+    else
+    {
+        double d1x, d1y, d1z;
+        bool cut;
+        const double m = line_m<PREC, true>(g, px, py, pz, tol, tol2, d1x, d1y, d1z, cut);
+        const double cx = fma(d1y, g.ez, -(d1z * g.ey));
+        const double cy = fma(d1z, g.ex, -(d1x * g.ez));
+        const double cz = fma(d1x, g.ey, -(d1y * g.ex));
+        const double nc = fma(cx, nx, fma(cy, ny, cz * nz));
+        return cut ? 0.0 : m * nc;
     }
 }
 

@@ -7,6 +7,7 @@
 #include <cstdio>
 #include <mutex>
 #include <string>
+#include <vector>
 
 #include "../../include/pvl.h"
 
@@ -204,4 +205,23 @@ struct pvl_mesh
     // line -> (surface, sign) transpose, for line circulations
     uint32_t *d_line_offsets = nullptr; // [n_lines+1]
     uint32_t *d_line_surfs = nullptr;   // [offsets[n_surfaces]] surface | kReversed
+    // assembly program (pvl::build_assembly_program): per run of kAsmSurfaces surfaces
+    uint32_t *d_asm_off = nullptr; // [n_runs + 1] entry offsets
+    uint32_t *d_asm = nullptr;     // [2 * entries] (line | kReversed, op)
 };
+
+namespace pvl
+{
+// The assembly (induction_matrix3) walks its surfaces in runs of kAsmSurfaces consecutive
+// surfaces, one run per wave.  Inside a run each line's n . K is evaluated once and kept in
+// one of kAsmSlots per-lane LDS slots while a later surface of the run still needs it.  The
+// program lists, per run and in CSR order, one entry per surface line: word 0 the line with
+// the orientation bit, word 1 the operation (kAsmEval / kAsmEvalStore / kAsmLoad / kAsmNone,
+// the slot in bits 2-4, kAsmEnd on a surface's last entry; a surface without lines gets one
+// kAsmNone entry).  Built on the host once per mesh, in O(entries).
+constexpr unsigned kAsmSurfaces = 8;
+constexpr unsigned kAsmSlots = 4;
+constexpr uint32_t kAsmEval = 0, kAsmEvalStore = 1, kAsmLoad = 2, kAsmNone = 3, kAsmEnd = 1u << 8;
+void build_assembly_program(unsigned n_surfaces, const uint32_t *offsets, const uint32_t *entries,
+                            std::vector<uint32_t> &run_off, std::vector<uint32_t> &prog);
+} // namespace pvl

@@ -192,6 +192,98 @@ int pvl_wake_topology(unsigned n_strips, unsigned line_rows, unsigned *n_lines_o
     return PVL_OK;
 }
 
+} // extern "C"
+
+// Assembly program of a mesh (see kAsmSurfaces): per run of consecutive surfaces, entries in
+// CSR order; a line used again later in the run is kept in an LDS slot, slots evicted by
+// farthest next use (a line whose slot was taken is evaluated again).
+void pvl::build_assembly_program(unsigned n_surfaces, const uint32_t *offsets, const uint32_t *entries,
+                                 std::vector<uint32_t> &run_off, std::vector<uint32_t> &prog)
+{
+    const unsigned n_runs = (n_surfaces + kAsmSurfaces - 1) / kAsmSurfaces;
+    run_off.assign(n_runs + 1, 0);
+    prog.clear();
+    prog.reserve(2 * ((size_t)offsets[n_surfaces] + n_surfaces));
+    constexpr uint32_t kNever = 0xFFFFFFFFu;
+    std::vector<uint32_t> line_of, next_use;
+    std::unordered_map<uint32_t, uint32_t> last_seen;
+    for (unsigned r = 0; r < n_runs; ++r)
+    {
+        const unsigned s0 = r * kAsmSurfaces, s1 = std::min(n_surfaces, s0 + kAsmSurfaces);
+        const uint32_t k0 = offsets[s0], k1 = offsets[s1];
+        const uint32_t n = k1 - k0;
+        line_of.resize(n);
+        next_use.assign(n, kNever);
+        last_seen.clear();
+        for (uint32_t i = 0; i < n; ++i)
+            line_of[i] = entries[k0 + i] & kInvalidId;
+        for (uint32_t i = n; i-- > 0;)
+        {
+            auto it = last_seen.find(line_of[i]);
+            next_use[i] = it == last_seen.end() ? kNever : it->second;
+            last_seen[line_of[i]] = i;
+        }
+        uint32_t slot_line[kAsmSlots], slot_next[kAsmSlots];
+        for (unsigned q = 0; q < kAsmSlots; ++q)
+            slot_line[q] = kNever, slot_next[q] = kNever;
+        for (unsigned s = s0; s < s1; ++s)
+        {
+            if (offsets[s + 1] == offsets[s])
+            {
+                prog.push_back(0);
+                prog.push_back(kAsmNone | kAsmEnd);
+                continue;
+            }
+            for (uint32_t k = offsets[s]; k < offsets[s + 1]; ++k)
+            {
+                const uint32_t i = k - k0, l = line_of[i];
+                uint32_t op = kAsmEval;
+                int held = -1;
+                for (unsigned q = 0; q < kAsmSlots; ++q)
+                    if (slot_line[q] == l)
+                        held = (int)q;
+                if (held >= 0)
+                {
+                    op = kAsmLoad | ((uint32_t)held << 2);
+                    slot_next[held] = next_use[i];
+                    if (next_use[i] == kNever)
+                        slot_line[held] = kNever;
+                }
+                else if (next_use[i] != kNever)
+                {
+                    // a free slot, else the one whose line is needed last (if later than this one)
+                    int q_use = -1;
+                    for (unsigned q = 0; q < kAsmSlots && q_use < 0; ++q)
+                        if (slot_line[q] == kNever)
+                            q_use = (int)q;
+                    if (q_use < 0)
+                    {
+                        unsigned far = 0;
+                        for (unsigned q = 1; q < kAsmSlots; ++q)
+                            if (slot_next[q] > slot_next[far])
+                                far = q;
+                        if (slot_next[far] > next_use[i])
+                            q_use = (int)far;
+                    }
+                    if (q_use >= 0)
+                    {
+                        op = kAsmEvalStore | ((uint32_t)q_use << 2);
+                        slot_line[q_use] = l;
+                        slot_next[q_use] = next_use[i];
+                    }
+                }
+                if (k + 1 == offsets[s + 1])
+                    op |= kAsmEnd;
+                prog.push_back(entries[k]);
+                prog.push_back(op);
+            }
+        }
+        run_off[r + 1] = (uint32_t)(prog.size() / 2);
+    }
+}
+
+extern "C" {
+
 int pvl_mesh_create(pvl_mesh **out, unsigned n_points, unsigned n_lines, const uint32_t *lines, unsigned n_surfaces,
                     const uint32_t *surface_offsets, const uint32_t *surface_lines)
 {
@@ -275,6 +367,14 @@ int pvl_mesh_create(pvl_mesh **out, unsigned n_points, unsigned n_lines, const u
         st = up(&m->d_line_offsets, loff.data(), n_lines + 1);
     if (!st)
         st = up(&m->d_line_surfs, lsurf.data(), n_entries);
+    if (!st)
+    {
+        std::vector<uint32_t> run_off, prog;
+        build_assembly_program(n_surfaces, surface_offsets, surface_lines, run_off, prog);
+        st = up(&m->d_asm_off, run_off.data(), run_off.size());
+        if (!st)
+            st = up(&m->d_asm, prog.data(), prog.size());
+    }
     if (st)
     {
         pvl_mesh_destroy(m);
@@ -295,6 +395,8 @@ int pvl_mesh_destroy(pvl_mesh *m)
     (void)hipFree(m->d_entries);
     (void)hipFree(m->d_line_offsets);
     (void)hipFree(m->d_line_surfs);
+    (void)hipFree(m->d_asm_off);
+    (void)hipFree(m->d_asm);
     delete m;
     return PVL_OK;
 }

@@ -1,11 +1,14 @@
 // Host-code sanitizer run (AddressSanitizer + UndefinedBehaviorSanitizer), CPU only.
 //
 // Drives the library's host topology builders (pvl_mesh_from_elements, pvl_mesh_dual,
-// pvl_wake_topology in pyvl_amd/csrc/topology.hip) and the oracle's C restatement
+// pvl_wake_topology and the assembly program builder in pyvl_amd/csrc/topology.hip; the
+// program is run on the CPU against the oracle's induction_matrix3) and the oracle's C restatement
 // (oracle/pvl_oracle.c: mesh construction, dual, line induction, surface reductions, point
 // velocity, line forces, LU) over seeded random meshes -- quads, triangles, mixed and
 // degenerate elements, shared and boundary edges -- and checks that both topologies agree.
 // Built by `make -C pyvl_amd/csrc sanitize`; no GPU call is made.
+#include <algorithm>
+#include <cmath>
 #include <cstdint>
 #include <cstdio>
 #include <cstdlib>
@@ -30,6 +33,81 @@ void orc_point_velocity(unsigned, const uint32_t *, unsigned, const uint32_t *, 
 void orc_line_forces(unsigned, const uint32_t *, const uint32_t *, const double *, const double *, const double *,
                      double *);
 int orc_lu_factor(unsigned, double *, int *);
+}
+
+// The assembly program builder (pyvl_amd/csrc/topology.hip; operations as in pvl_internal.hpp:
+// kind in bits 0-1 -- 0 evaluate, 1 evaluate and store, 2 load, 3 none --, slot in bits 2-4,
+// bit 8 on a surface's last entry).
+namespace pvl
+{
+void build_assembly_program(unsigned n_surfaces, const uint32_t *offsets, const uint32_t *entries,
+                            std::vector<uint32_t> &run_off, std::vector<uint32_t> &prog);
+}
+
+constexpr unsigned kAsmSurfaces = 8; // surfaces per run, as pvl::kAsmSurfaces
+
+// Runs the program as the kernel does, for one control point, with the per-line values
+// v[line] (n . K of that line at the point): slots hold (line, value); a load must find its
+// own line.  Returns the per-surface sums, or an empty vector when the program is malformed.
+static std::vector<double> run_program(unsigned n_el, const std::vector<uint32_t> &o, const std::vector<uint32_t> &e,
+                                       const std::vector<uint32_t> &run_off, const std::vector<uint32_t> &prog,
+                                       const std::vector<double> &v)
+{
+    const unsigned n_runs = (unsigned)run_off.size() - 1;
+    const unsigned per_run = kAsmSurfaces;
+    if (n_runs != (n_el + per_run - 1) / per_run)
+        return {};
+    std::vector<double> out;
+    for (unsigned r = 0; r < n_runs; ++r)
+    {
+        uint32_t slot_line[8];
+        double slot_val[8];
+        for (int q = 0; q < 8; ++q)
+            slot_line[q] = 0xFFFFFFFFu, slot_val[q] = 0.0;
+        double acc = 0.0;
+        unsigned ends = 0, s = r * per_run;
+        uint32_t k_csr = o[s];
+        for (uint32_t k = run_off[r]; k < run_off[r + 1]; ++k)
+        {
+            const uint32_t word = prog[2 * k], op = prog[2 * k + 1];
+            const uint32_t kind = op & 3u, slot = (op >> 2) & 7u, line = word & 0x7FFFFFFFu;
+            double val = 0.0;
+            if (kind == 3)
+            {
+                if (o[s + ends] != o[s + ends + 1])
+                    return {};
+            }
+            else
+            {
+                if (word != e[k_csr++])
+                    return {};
+                if (kind == 2)
+                {
+                    if (slot_line[slot] != line)
+                        return {};
+                    val = slot_val[slot];
+                }
+                else
+                {
+                    val = v[line];
+                    if (kind == 1)
+                        slot_line[slot] = line, slot_val[slot] = val;
+                }
+            }
+            acc = (word & 0x80000000u) ? acc - val : acc + val;
+            if (op & (1u << 8))
+            {
+                out.push_back(acc);
+                acc = 0.0;
+                ++ends;
+                if (s + ends < n_el && k_csr != o[s + ends])
+                    return {};
+            }
+        }
+        if (ends != std::min(per_run, n_el - r * per_run))
+            return {};
+    }
+    return out;
 }
 
 static int fail(const char *what, int k)
@@ -119,6 +197,32 @@ int main()
         orc_point_velocity(nl1, l1.data(), n_el, o1.data(), e1.data(), pos.data(), circ.data(), n_cpts, cpts.data(),
                            pv.data(), 1e-6);
         orc_line_forces(nl1, l1.data(), dl1.data(), circ.data(), pos.data(), vel.data(), lf.data());
+        // the assembly program reproduces every surface's sum for every control point
+        {
+            std::vector<uint32_t> run_off, prog;
+            pvl::build_assembly_program(n_el, o1.data(), e1.data(), run_off, prog);
+            for (unsigned c = 0; c < n_cpts && !bad; ++c)
+            {
+                std::vector<double> v(nl1 ? nl1 : 1);
+                for (unsigned l = 0; l < nl1; ++l)
+                {
+                    const double *kv = &lk[3 * ((size_t)c * nl1 + l)];
+                    v[l] = kv[0] * nrm[3 * c] + kv[1] * nrm[3 * c + 1] + kv[2] * nrm[3 * c + 2];
+                }
+                const std::vector<double> got = run_program(n_el, o1, e1, run_off, prog, v);
+                if (got.size() != n_el)
+                {
+                    bad |= fail("assembly program structure", k);
+                    break;
+                }
+                for (unsigned s = 0; s < n_el && !bad; ++s)
+                {
+                    const double want = a3[(size_t)c * n_el + s];
+                    if (std::fabs(got[s] - want) > 1e-12 * (1.0 + std::fabs(want)))
+                        bad |= fail("assembly program sum", k);
+                }
+            }
+        }
         const unsigned n = 5 + k % 20;
         std::vector<double> a((size_t)n * n);
         std::vector<int> piv(n);
