@@ -121,8 +121,8 @@ int point_velocity_reduce(unsigned n_pts, unsigned n_chunks, const double *parti
 // of nodes; a piece is a run of up to kChainPiece consecutive segments of one chain, starting
 // at node start_node, whose segment records are segs[first_seg .. first_seg + n_seg).  A
 // piece with row1 > 0 runs streamwise from node row row1 - 1 and only its first cr - (row1 - 1)
-// segments are active at cr filled rows.  Walking a piece, each node's p - r and |p - r|^2
-// are formed once and serve both segments that meet there.
+// segments are active at cr filled rows.  Walking a piece, each node's p - r (and, in the
+// reference form, |p - r|^2) is formed once and serves both segments that meet there.
 constexpr unsigned kChainPiece = 32;
 struct ChainPiece
 {
