@@ -144,7 +144,7 @@ __global__ void __launch_bounds__(256) panel_kernel(T *__restrict__ P, int lda, 
     extern __shared__ __align__(16) unsigned char smem_raw[];
     constexpr int LDP = NB + 1;
     T *L = (T *)smem_raw;
-    __shared__ T prow[NB];
+    __shared__ T prow[NB], srowj[NB];
     __shared__ double red_v[4];
     __shared__ int red_p[4];
     __shared__ int s_pos, s_grp;
@@ -211,7 +211,10 @@ __global__ void __launch_bounds__(256) panel_kernel(T *__restrict__ P, int lda, 
             grid_barrier(&X->counter, base + (unsigned)G * (unsigned)(j + 1), &status[1]);
         else
             __syncthreads();
-        // 4. winner (identical in every group)
+        // 4. winner (identical in every group); meanwhile the second wave fetches row j, the
+        // pivot row whenever no interchange is due (the usual case), in the same round trip
+        if (tid >= 64 && tid < 64 + jb)
+            srowj[tid - 64] = xload(&X->rowj[par][tid - 64]);
         if (tid < 64)
         {
             double v = -1.0;
@@ -240,8 +243,9 @@ __global__ void __launch_bounds__(256) panel_kernel(T *__restrict__ P, int lda, 
         const bool none = (p == INT_MAX); // every candidate NaN: keep row j in place
         if (none)
             p = j;
+        // the winner's published row equals row j when p == j (the same values published twice)
         for (int c = tid; c < jb; c += 256)
-            prow[c] = none ? xload(&X->rowj[par][c]) : xload(&X->crow[par][wg][c]);
+            prow[c] = (p == j) ? srowj[c] : xload(&X->crow[par][wg][c]);
         __syncthreads();
         // 5. interchange
         const bool own_p = p >= r0 && p < r0 + nr;
@@ -252,7 +256,7 @@ __global__ void __launch_bounds__(256) panel_kernel(T *__restrict__ P, int lda, 
                     L[(j - r0) * LDP + c] = prow[c];
             if (own_p)
                 for (int c = tid; c < jb; c += 256)
-                    L[(p - r0) * LDP + c] = xload(&X->rowj[par][c]);
+                    L[(p - r0) * LDP + c] = srowj[c];
         }
         if (tid == 0)
             spiv[j] = row_base + p; // global store deferred: see the end of the kernel
