@@ -166,30 +166,33 @@ __global__ void __launch_bounds__(256) panel_kernel(T *__restrict__ P, int lda, 
     for (int j = 0; j < jb; ++j)
     {
         const int par = j & 1;
-        // 1. local candidate
-        double bv = -1.0;
-        int bp = INT_MAX;
-        for (int r = tid; r < nr; r += 256)
+        // 1. local candidate: per lane, then over the wave (DPP moves), then over the waves
         {
-            const int pos = r0 + r;
-            if (pos >= j)
+            unsigned long long key = 0;
+            unsigned row = 0xFFFFFFFFu;
+            for (int r = tid; r < nr; r += 256)
             {
-                const double v = fabs((double)L[r * LDP + j]);
-                if (better(v, pos, bv, bp))
-                    bv = v, bp = pos;
+                const int pos = r0 + r;
+                if (pos >= j)
+                {
+                    const double v = fabs((double)L[r * LDP + j]);
+                    const unsigned long long kk = pivot_key(v);
+                    if (v == v && (kk > key || (kk == key && (unsigned)pos < row)))
+                        key = kk, row = (unsigned)pos;
+                }
+            }
+            unsigned long long km;
+            const unsigned wrow = wave_argmax(key, row, &km);
+            if ((tid & 63) == 0)
+            {
+                const bool none = wrow == 0xFFFFFFFFu;
+                red_v[tid >> 6] = none ? -1.0 : __longlong_as_double((long long)km);
+                red_p[tid >> 6] = none ? INT_MAX : (int)wrow;
             }
         }
-        for (int off = 32; off > 0; off >>= 1)
-        {
-            const double ov = __shfl_xor(bv, off);
-            const int op = __shfl_xor(bp, off);
-            if (better(ov, op, bv, bp))
-                bv = ov, bp = op;
-        }
-        if ((tid & 63) == 0)
-            red_v[tid >> 6] = bv, red_p[tid >> 6] = bp;
         __syncthreads();
-        bv = red_v[0], bp = red_p[0];
+        double bv = red_v[0];
+        int bp = red_p[0];
         for (int w = 1; w < 4; ++w)
             if (better(red_v[w], red_p[w], bv, bp))
                 bv = red_v[w], bp = red_p[w];
@@ -217,25 +220,25 @@ __global__ void __launch_bounds__(256) panel_kernel(T *__restrict__ P, int lda, 
             srowj[tid - 64] = xload(&X->rowj[par][tid - 64]);
         if (tid < 64)
         {
-            double v = -1.0;
-            int p = INT_MAX, wg = 0;
+            // candidates (|a| >= 0, or -1 with position INT_MAX for a group without one)
+            unsigned long long key = 0;
+            unsigned row = 0xFFFFFFFFu;
+            int bq = 0;
             for (int q = tid; q < G; q += 64)
             {
                 const double qv = xload(&X->cval[par][q]);
                 const int qp = xload_i(&X->cpos[par][q]);
-                if (better(qv, qp, v, p))
-                    v = qv, p = qp, wg = q;
+                const unsigned long long kk = pivot_key(qv);
+                if (qp != INT_MAX && qv >= 0.0 && (kk > key || (kk == key && (unsigned)qp < row)))
+                    key = kk, row = (unsigned)qp, bq = q;
             }
-            for (int off = 32; off > 0; off >>= 1)
-            {
-                const double ov = __shfl_xor(v, off);
-                const int op = __shfl_xor(p, off);
-                const int og = __shfl_xor(wg, off);
-                if (better(ov, op, v, p))
-                    v = ov, p = op, wg = og;
-            }
+            unsigned long long km;
+            const unsigned wrow = wave_argmax(key, row, &km);
+            const unsigned long long who = __ballot(wrow != 0xFFFFFFFFu && row == wrow);
+            const int wl = who ? (int)__builtin_ctzll(who) : 0;
+            const int wg = __builtin_amdgcn_readlane(bq, wl);
             if (tid == 0)
-                s_pos = p, s_grp = wg;
+                s_pos = wrow == 0xFFFFFFFFu ? INT_MAX : (int)wrow, s_grp = wg;
         }
         __syncthreads();
         int p = s_pos;
