@@ -163,127 +163,242 @@ __global__ void __launch_bounds__(256) panel_kernel(T *__restrict__ P, int lda, 
     }
     __syncthreads();
 
-    for (int j = 0; j < jb; ++j)
+    // Columns in sub-blocks of KB: a sub-block's columns live in registers (one row per thread)
+    // and are updated column by column; the panel's columns right of the sub-block are
+    // updated once per sub-block from the sub-block's pivot rows (U12 of the sub-block,
+    // formed by every group from the published pivot rows) -- per element the same FMAs in the
+    // same order as the column-by-column sweep, so factors and pivots are unchanged, with an
+    // eighth of the LDS traffic.  Rows published for the exchange carry the sub-block's
+    // columns from registers and the columns right of it as they stand (not yet updated by the
+    // sub-block): every consumer (the interchange, U12) treats them the same way.
+    constexpr int KB = 8;
+    __shared__ T ucur[KB][NB]; // the sub-block's pivot rows as published, then U12 in place
+    __shared__ T cand_sub[KB], rowj_sub[KB];
+    const int r = tid; // this thread's row (R <= 256 rows per group)
+    const bool has_row = r < nr;
+    const int pos = r0 + r;
+    for (int j0 = 0; j0 < jb; j0 += KB)
     {
-        const int par = j & 1;
-        // 1. local candidate: per lane, then over the wave (DPP moves), then over the waves
+        const int kb = min(KB, jb - j0);
+        T v[KB];
+#pragma unroll
+        for (int t = 0; t < KB; ++t)
+            v[t] = (has_row && t < kb) ? L[r * LDP + j0 + t] : T(0);
+        unsigned zm = 0; // bit t: column j0 + t had an exactly-zero pivot (no update applied)
+#pragma unroll
+        for (int t = 0; t < KB; ++t)
         {
-            unsigned long long key = 0;
-            unsigned row = 0xFFFFFFFFu;
-            for (int r = tid; r < nr; r += 256)
+            if (t >= kb)
+                break;
+            const int j = j0 + t;
+            const int par = j & 1;
+            // 1. local candidate: per lane, then over the wave (DPP moves), then over the waves
             {
-                const int pos = r0 + r;
-                if (pos >= j)
+                unsigned long long key = 0;
+                unsigned row = 0xFFFFFFFFu;
+                if (has_row && pos >= j)
                 {
-                    const double v = fabs((double)L[r * LDP + j]);
-                    const unsigned long long kk = pivot_key(v);
-                    if (v == v && (kk > key || (kk == key && (unsigned)pos < row)))
-                        key = kk, row = (unsigned)pos;
+                    const double vv = fabs((double)v[t]);
+                    if (vv == vv)
+                        key = pivot_key(vv), row = (unsigned)pos;
+                }
+                unsigned long long km;
+                const unsigned wrow = wave_argmax(key, row, &km);
+                if ((tid & 63) == 0)
+                {
+                    const bool none = wrow == 0xFFFFFFFFu;
+                    red_v[tid >> 6] = none ? -1.0 : __longlong_as_double((long long)km);
+                    red_p[tid >> 6] = none ? INT_MAX : (int)wrow;
                 }
             }
-            unsigned long long km;
-            const unsigned wrow = wave_argmax(key, row, &km);
-            if ((tid & 63) == 0)
-            {
-                const bool none = wrow == 0xFFFFFFFFu;
-                red_v[tid >> 6] = none ? -1.0 : __longlong_as_double((long long)km);
-                red_p[tid >> 6] = none ? INT_MAX : (int)wrow;
-            }
-        }
-        __syncthreads();
-        double bv = red_v[0];
-        int bp = red_p[0];
-        for (int w = 1; w < 4; ++w)
-            if (better(red_v[w], red_p[w], bv, bp))
-                bv = red_v[w], bp = red_p[w];
-        // 2. publish candidate, its row, and the current row j if owned
-        if (tid == 0)
-        {
-            xstore(&X->cval[par][g], bv);
-            xstore_i(&X->cpos[par][g], bp);
-        }
-        if (bp != INT_MAX)
-            for (int c = tid; c < jb; c += 256)
-                xstore(&X->crow[par][g][c], L[(bp - r0) * LDP + c]);
-        const bool own_j = j >= r0 && j < r0 + nr;
-        if (own_j)
-            for (int c = tid; c < jb; c += 256)
-                xstore(&X->rowj[par][c], L[(j - r0) * LDP + c]);
-        // 3. barrier
-        if (G > 1)
-            grid_barrier(&X->counter, base + (unsigned)G * (unsigned)(j + 1), &status[1]);
-        else
             __syncthreads();
-        // 4. winner (identical in every group); meanwhile the second wave fetches row j, the
-        // pivot row whenever no interchange is due (the usual case), in the same round trip
-        if (tid >= 64 && tid < 64 + jb)
-            srowj[tid - 64] = xload(&X->rowj[par][tid - 64]);
-        if (tid < 64)
-        {
-            // candidates (|a| >= 0, or -1 with position INT_MAX for a group without one)
-            unsigned long long key = 0;
-            unsigned row = 0xFFFFFFFFu;
-            int bq = 0;
-            for (int q = tid; q < G; q += 64)
+            double bv = red_v[0];
+            int bp = red_p[0];
+            for (int w = 1; w < 4; ++w)
+                if (better(red_v[w], red_p[w], bv, bp))
+                    bv = red_v[w], bp = red_p[w];
+            // the candidate's and row j's sub-block columns, from their owners' registers
+            const bool own_j = j >= r0 && j < r0 + nr;
+            if (has_row && pos == bp)
             {
-                const double qv = xload(&X->cval[par][q]);
-                const int qp = xload_i(&X->cpos[par][q]);
-                const unsigned long long kk = pivot_key(qv);
-                if (qp != INT_MAX && qv >= 0.0 && (kk > key || (kk == key && (unsigned)qp < row)))
-                    key = kk, row = (unsigned)qp, bq = q;
+#pragma unroll
+                for (int i = 0; i < KB; ++i)
+                    cand_sub[i] = v[i];
             }
-            unsigned long long km;
-            const unsigned wrow = wave_argmax(key, row, &km);
-            const unsigned long long who = __ballot(wrow != 0xFFFFFFFFu && row == wrow);
-            const int wl = who ? (int)__builtin_ctzll(who) : 0;
-            const int wg = __builtin_amdgcn_readlane(bq, wl);
+            if (has_row && pos == j)
+            {
+#pragma unroll
+                for (int i = 0; i < KB; ++i)
+                    rowj_sub[i] = v[i];
+            }
+            __syncthreads();
+            // 2. publish candidate, its row, and the current row j if owned
             if (tid == 0)
-                s_pos = wrow == 0xFFFFFFFFu ? INT_MAX : (int)wrow, s_grp = wg;
-        }
-        __syncthreads();
-        int p = s_pos;
-        const int wg = s_grp;
-        const bool none = (p == INT_MAX); // every candidate NaN: keep row j in place
-        if (none)
-            p = j;
-        // the winner's published row equals row j when p == j (the same values published twice)
-        for (int c = tid; c < jb; c += 256)
-            prow[c] = (p == j) ? srowj[c] : xload(&X->crow[par][wg][c]);
-        __syncthreads();
-        // 5. interchange
-        const bool own_p = p >= r0 && p < r0 + nr;
-        if (p != j)
-        {
+            {
+                xstore(&X->cval[par][g], bv);
+                xstore_i(&X->cpos[par][g], bp);
+            }
+            if (bp != INT_MAX)
+                for (int c = tid; c < jb; c += 256)
+                    xstore(&X->crow[par][g][c],
+                           (c >= j0 && c < j0 + kb) ? cand_sub[c - j0] : L[(bp - r0) * LDP + c]);
             if (own_j)
                 for (int c = tid; c < jb; c += 256)
-                    L[(j - r0) * LDP + c] = prow[c];
-            if (own_p)
-                for (int c = tid; c < jb; c += 256)
-                    L[(p - r0) * LDP + c] = srowj[c];
-        }
-        if (tid == 0)
-            spiv[j] = row_base + p; // global store deferred: see the end of the kernel
-        __syncthreads();
-        // 6. scale by the reciprocal pivot and update the trailing panel columns
-        const T pv = prow[j];
-        if (pv != T(0))
-        {
-            const T inv = T(1) / pv;
-            for (int r = tid; r < nr; r += 256)
+                    xstore(&X->rowj[par][c], (c >= j0 && c < j0 + kb) ? rowj_sub[c - j0] : L[(j - r0) * LDP + c]);
+            // 3. barrier
+            if (G > 1)
+                grid_barrier(&X->counter, base + (unsigned)G * (unsigned)(j + 1), &status[1]);
+            else
+                __syncthreads();
+            // 4. winner (identical in every group); meanwhile the second wave fetches row j, the
+            // pivot row whenever no interchange is due (the usual case), in the same round trip
+            if (tid >= 64 && tid < 64 + jb)
+                srowj[tid - 64] = xload(&X->rowj[par][tid - 64]);
+            if (tid < 64)
             {
-                if (r0 + r > j)
+                // candidates (|a| >= 0, or -1 with position INT_MAX for a group without one)
+                unsigned long long key = 0;
+                unsigned row = 0xFFFFFFFFu;
+                int bq = 0;
+                for (int q = tid; q < G; q += 64)
                 {
-                    T *row = L + r * LDP;
-                    const T l = row[j] * inv;
-                    row[j] = l;
-                    for (int c = j + 1; c < jb; ++c)
-                        row[c] = fma(-l, prow[c], row[c]);
+                    const double qv = xload(&X->cval[par][q]);
+                    const int qp = xload_i(&X->cpos[par][q]);
+                    const unsigned long long kk = pivot_key(qv);
+                    if (qp != INT_MAX && qv >= 0.0 && (kk > key || (kk == key && (unsigned)qp < row)))
+                        key = kk, row = (unsigned)qp, bq = q;
+                }
+                unsigned long long km;
+                const unsigned wrow = wave_argmax(key, row, &km);
+                const unsigned long long who = __ballot(wrow != 0xFFFFFFFFu && row == wrow);
+                const int wl = who ? (int)__builtin_ctzll(who) : 0;
+                const int wgq = __builtin_amdgcn_readlane(bq, wl);
+                if (tid == 0)
+                    s_pos = wrow == 0xFFFFFFFFu ? INT_MAX : (int)wrow, s_grp = wgq;
+            }
+            __syncthreads();
+            int p = s_pos;
+            const int wg = s_grp;
+            const bool none = (p == INT_MAX); // every candidate NaN: keep row j in place
+            if (none)
+                p = j;
+            // the winner's published row equals row j when p == j (the same values published twice)
+            for (int c = tid; c < jb; c += 256)
+                prow[c] = (p == j) ? srowj[c] : xload(&X->crow[par][wg][c]);
+            __syncthreads();
+            // 5. interchange: LDS rows outside the sub-block, the owners' registers inside
+            const bool own_p = p >= r0 && p < r0 + nr;
+            if (p != j)
+            {
+                if (own_j)
+                    for (int c = tid; c < jb; c += 256)
+                        if (c < j0 || c >= j0 + kb)
+                            L[(j - r0) * LDP + c] = prow[c];
+                if (own_p)
+                    for (int c = tid; c < jb; c += 256)
+                        if (c < j0 || c >= j0 + kb)
+                            L[(p - r0) * LDP + c] = srowj[c];
+                if (has_row && pos == j)
+                {
+#pragma unroll
+                    for (int i = 0; i < KB; ++i)
+                        v[i] = (i < kb) ? prow[j0 + i] : v[i];
+                }
+                else if (has_row && pos == p)
+                {
+#pragma unroll
+                    for (int i = 0; i < KB; ++i)
+                        v[i] = (i < kb) ? srowj[j0 + i] : v[i];
+                }
+            }
+            for (int c = tid; c < jb; c += 256)
+                ucur[t][c] = prow[c];
+            if (tid == 0)
+                spiv[j] = row_base + p; // global store deferred: see the end of the kernel
+            // 6. scale by the reciprocal pivot and update the sub-block's columns (registers)
+            const T pv = prow[j];
+            if (pv != T(0))
+            {
+                const T inv = T(1) / pv;
+                if (has_row && pos > j)
+                {
+                    const T l = v[t] * inv;
+                    v[t] = l;
+#pragma unroll
+                    for (int t2 = t + 1; t2 < KB; ++t2)
+                        if (t2 < kb)
+                            v[t2] = fma(-l, prow[j0 + t2], v[t2]);
+                }
+            }
+            else
+            {
+                zm |= 1u << t;
+                if (tid == 0 && first_zero == 0)
+                    first_zero = row_base + j + 1; // first exactly-zero pivot (1-based global column)
+            }
+        }
+        // 7. the sub-block's columns back to LDS; U12 of the sub-block for the columns right
+        // of it, pivot row by pivot row in elimination order
+        if (has_row)
+        {
+#pragma unroll
+            for (int t = 0; t < KB; ++t)
+                if (t < kb)
+                    L[r * LDP + j0 + t] = v[t];
+        }
+        __syncthreads();
+        for (int c = j0 + kb + tid; c < jb; c += 256)
+        {
+            T u[KB];
+#pragma unroll
+            for (int k = 0; k < KB; ++k)
+            {
+                if (k < kb)
+                {
+                    T x = ucur[k][c];
+#pragma unroll
+                    for (int i = 0; i < KB; ++i)
+                        if (i < k && !((zm >> i) & 1u))
+                            x = fma(-ucur[k][j0 + i], u[i], x);
+                    u[k] = x;
+                    ucur[k][c] = x;
                 }
             }
         }
-        else if (tid == 0 && first_zero == 0)
+        __syncthreads();
+        // 8. the columns right of the sub-block: rows below its pivots take its kb updates
+        // (multipliers in registers), its pivot rows their U12 rows
+        if (has_row && j0 + kb < jb)
         {
-            first_zero = row_base + j + 1; // first exactly-zero pivot (1-based global column)
+            if (pos >= j0 + kb)
+            {
+                // eight columns at a time: eight independent FMA chains (one wave per SIMD
+                // here, so a single chain would wait on every FMA's latency)
+                for (int cb = j0 + kb; cb < jb; cb += 8)
+                {
+                    T x[8];
+#pragma unroll
+                    for (int q = 0; q < 8; ++q)
+                        x[q] = (cb + q < jb) ? L[r * LDP + cb + q] : T(0);
+#pragma unroll
+                    for (int i = 0; i < KB; ++i)
+                        if (i < kb && !((zm >> i) & 1u))
+                        {
+#pragma unroll
+                            for (int q = 0; q < 8; ++q)
+                                x[q] = fma(-v[i], ucur[i][min(cb + q, jb - 1)], x[q]);
+                        }
+#pragma unroll
+                    for (int q = 0; q < 8; ++q)
+                        if (cb + q < jb)
+                            L[r * LDP + cb + q] = x[q];
+                }
+            }
+            else if (pos >= j0)
+            {
+                for (int c = j0 + kb; c < jb; ++c)
+                    L[r * LDP + c] = ucur[pos - j0][c];
+            }
         }
         __syncthreads();
     }
