@@ -7,9 +7,9 @@
 // same matrix, with piv[j] = row swapped with row j at step j (0-based, as scipy returns).
 //
 // Right-looking blocked algorithm, panel width NB = 64:
-//   panel_kernel   cooperative grid; each workgroup holds up to R panel rows in LDS; one
-//                  grid-wide exchange per column carries every group's pivot candidate
-//                  together with the candidate row, so a column costs one barrier.
+//   panel_kernel   cooperative grid; each workgroup holds up to R panel rows in LDS; per
+//                  column every group publishes its pivot candidate's row and then a tagged
+//                  record of the candidate, and reads every group's record (no barrier).
 //   laswp_kernel   applies the panel's interchanges to the other columns.
 //   trsm_kernel    U12 = L11^-1 A12 (one lane per column in registers, L11 in LDS).
 //   gemm_kernel    A22 -= L21 U12 on MFMA (v_mfma_f64_16x16x4f64 / v_mfma_f32_16x16x4f32).
@@ -93,46 +93,153 @@ __device__ __forceinline__ void xstore_i(int *p, int v)
     __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// Grid barrier on a monotonically increasing counter; bounded: on timeout sets *err.
-__device__ __forceinline__ void grid_barrier(unsigned *counter, unsigned target, int *err)
+__device__ __forceinline__ unsigned long long xload_u64(const unsigned long long *p)
 {
-    // every wave's exchange stores (device-coherent) complete; thread 0's release covers the
-    // workgroup
-    __builtin_amdgcn_s_waitcnt(0);
-    __syncthreads();
-    if (threadIdx.x == 0)
-    {
-        __hip_atomic_fetch_add(counter, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-        unsigned spins = 0;
-        while (__hip_atomic_load(counter, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < target)
-        {
-            __builtin_amdgcn_s_sleep(1);
-            if (++spins > kSpinLimit || __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
-            {
-                __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                break;
-            }
-        }
-    }
-    __syncthreads();
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void xstore_u64(unsigned long long *p, unsigned long long v)
+{
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ unsigned xload_u32(const unsigned *p)
+{
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void xstore_u32(unsigned *p, unsigned v)
+{
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// Bounded waits: a wait that has not ended after kWaitTicks of the 100 MHz real-time
+// counter (or once another wave has flagged a timeout) gives up and flags *err.
+constexpr unsigned long long kWaitTicks = 200000000ull; // 2 s
+__device__ __forceinline__ bool wait_expired(unsigned &spins, unsigned long long t0, const int *err)
+{
+    if ((++spins & 255u) != 0)
+        return false;
+    return wall_clock64() - t0 > kWaitTicks || xload_i(err) != 0;
+}
 
-// Exchange area for the panel kernel (double buffered by column parity).
+constexpr int kRecBytes = 256;
+
+// Exchange area for the panel kernel, double buffered by the parity of the exchange round
+// (one round per column).  There is no grid barrier: a group's record of a round is the
+// signal that its published rows of that round are complete, and every word of a record
+// carries a tag bit (bit 1 of the round number), so that a reader tells the record of this
+// round from the one two rounds earlier in the same buffer.  A group can run at most one
+// round ahead of the slowest (it needs every group's record of the round before), so two
+// buffers and one tag bit suffice.
+// Each record sits in a cache line of its own: every group polls every record, and records
+// packed into a few lines queue those loads (and the stores behind them) on one memory channel.
+struct alignas(kRecBytes) PanelSlot
+{
+    unsigned long long key; // bit 63: tag; bits 0-62: the candidate's |a| (double bits)
+    unsigned pos;           // bit 31: tag; bits 0-30: its panel position (INT_MAX: none)
+};
+struct PanelRec
+{
+    PanelSlot slot[2][kMaxGroups];
+};
 template <typename T> struct PanelX
 {
-    unsigned counter; // grid-barrier arrivals, monotonic over one factorisation
-    int pad[3];
-    double cval[2][kMaxGroups];
-    int cpos[2][kMaxGroups];
+    PanelRec rec;
     T crow[2][kMaxGroups][NB];
     T rowj[2][NB];
 };
 
 // ---------------------------------------------------------------------------------------
+// The panel's columns [c0, jb) of local rows [rb, nr) take a sub-block's update,
+//   x[r][c] -= sum_i L[r][j0 + i] * U[i][c]   (i = 0 .. kb-1 ascending, zero pivots skipped),
+// each element by the same FMAs in the same order as the column-by-column sweep.  A thread
+// takes TR rows x 8 columns: per FMA it reads a quarter (TR = 8) of the operands that one
+// row per thread with every U value broadcast from LDS reads, and the LDS, not the FMA
+// pipes, bounds this step.  Rows of a tile are strided by the number of row tiles, so the
+// lanes of a wave read consecutive rows (row stride 2 banks: conflict-free).  Loads clamp
+// their indices and stores outside the block go to row `sink` (one row past the group's
+// rows) or to the padding column NBW, so the code has no per-element branches; the
+// operands of step i + 1 are fetched while step i's FMAs run.
+template <typename T, int TR, int KB, int LDP, int NBW>
+__device__ __forceinline__ void subblock_update(T *__restrict__ L, const T (*__restrict__ U)[NBW], int rb, int nr,
+                                                int sink, int j0, int kb, int c0, int jb, unsigned zm)
+{
+    const int nrows = nr - rb, ncols = jb - c0;
+    const int nrt = (nrows + TR - 1) / TR, nct = (ncols + 7) / 8;
+    const int tid = threadIdx.x;
+    if (tid >= nrt * nct)
+        return;
+    const int rt = tid % nrt, ct = tid / nrt;
+    const int cb = c0 + 8 * ct;
+    int rl[TR], rs[TR], cl[8], cs[8];
+#pragma unroll
+    for (int q = 0; q < TR; ++q)
+    {
+        const int r = rb + rt + nrt * q;
+        rl[q] = min(r, nr - 1) * LDP;
+        rs[q] = (r < nr ? r : sink) * LDP;
+    }
+#pragma unroll
+    for (int cc = 0; cc < 8; ++cc)
+    {
+        cl[cc] = min(cb + cc, jb - 1);
+        cs[cc] = cb + cc < jb ? cb + cc : NBW;
+    }
+    T x[TR][8];
+#pragma unroll
+    for (int q = 0; q < TR; ++q)
+#pragma unroll
+        for (int cc = 0; cc < 8; ++cc)
+            x[q][cc] = L[rl[q] + cl[cc]];
+    T mu[TR], u[8];
+#pragma unroll
+    for (int q = 0; q < TR; ++q)
+        mu[q] = L[rl[q] + j0];
+#pragma unroll
+    for (int cc = 0; cc < 8; ++cc)
+        u[cc] = U[0][cl[cc]];
+#pragma unroll
+    for (int i = 0; i < KB; ++i)
+    {
+        T mn[TR], un[8];
+        if (i + 1 < KB)
+        {
+            const int in = min(i + 1, kb - 1);
+#pragma unroll
+            for (int q = 0; q < TR; ++q)
+                mn[q] = L[rl[q] + j0 + in];
+#pragma unroll
+            for (int cc = 0; cc < 8; ++cc)
+                un[cc] = U[in][cl[cc]];
+        }
+        if (i < kb && !((zm >> i) & 1u))
+        {
+#pragma unroll
+            for (int q = 0; q < TR; ++q)
+#pragma unroll
+                for (int cc = 0; cc < 8; ++cc)
+                    x[q][cc] = fma(-mu[q], u[cc], x[q][cc]);
+        }
+        if (i + 1 < KB)
+        {
+#pragma unroll
+            for (int q = 0; q < TR; ++q)
+                mu[q] = mn[q];
+#pragma unroll
+            for (int cc = 0; cc < 8; ++cc)
+                u[cc] = un[cc];
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < TR; ++q)
+#pragma unroll
+        for (int cc = 0; cc < 8; ++cc)
+            L[rs[q] + cs[cc]] = x[q][cc];
+}
+
+// ---------------------------------------------------------------------------------------
 // Panel factorisation of P (m x jb, row stride lda), rows k.. of the matrix.  Group g owns
-// panel positions [g*R, g*R+R).  Per column j: local argmax over positions >= j, publish
-// (|value|, position, row) plus the current row j, grid barrier, every group picks the
+// panel positions g, g + G, g + 2G, ... (at most R of them).  Per column j: local argmax
+// over positions >= j; publish the candidate's row and the current row j, then the
+// candidate's record (|value|, position); read every group's record; every group picks the
 // same winner (largest |a|, first position on ties: LAPACK idamax), swaps, scales by the
 // reciprocal pivot and applies the rank-1 update to its rows.
 template <typename T>
@@ -154,12 +261,14 @@ __global__ void __launch_bounds__(256) panel_kernel(T *__restrict__ P, int lda, 
     int first_zero = 0;
 
     const int tid = threadIdx.x, g = blockIdx.x, G = gridDim.x;
-    const int r0 = g * R;
-    const int nr = max(0, min(m, r0 + R) - r0);
+    // rows dealt cyclically: group g holds panel positions g, g + G, g + 2G, ... (at most R),
+    // so row j, its owner's extra publishing and the sub-block's pivot rows rotate over the
+    // groups instead of all falling to the first
+    const int nr = g < m ? (m - 1 - g) / G + 1 : 0;
     for (int idx = tid; idx < nr * jb; idx += 256)
     {
         const int r = idx / jb, c = idx - r * jb;
-        L[r * LDP + c] = P[(size_t)(r0 + r) * lda + c];
+        L[r * LDP + c] = P[(size_t)(g + r * G) * lda + c];
     }
     __syncthreads();
 
@@ -176,7 +285,7 @@ __global__ void __launch_bounds__(256) panel_kernel(T *__restrict__ P, int lda, 
     __shared__ T cand_sub[KB], rowj_sub[KB];
     const int r = tid; // this thread's row (R <= 256 rows per group)
     const bool has_row = r < nr;
-    const int pos = r0 + r;
+    const int pos = g + r * G;
     for (int j0 = 0; j0 < jb; j0 += KB)
     {
         const int kb = min(KB, jb - j0);
@@ -191,7 +300,11 @@ __global__ void __launch_bounds__(256) panel_kernel(T *__restrict__ P, int lda, 
             if (t >= kb)
                 break;
             const int j = j0 + t;
-            const int par = j & 1;
+            const unsigned rnd = base + (unsigned)j; // exchange round
+            const int par = (int)(rnd & 1u);
+            const unsigned tagbit = (rnd >> 1) & 1u;
+            const unsigned long long ktag = (unsigned long long)tagbit << 63;
+            const unsigned ptag = tagbit << 31;
             // 1. local candidate: per lane, then over the wave (DPP moves), then over the waves
             {
                 unsigned long long key = 0;
@@ -218,7 +331,8 @@ __global__ void __launch_bounds__(256) panel_kernel(T *__restrict__ P, int lda, 
                 if (better(red_v[w], red_p[w], bv, bp))
                     bv = red_v[w], bp = red_p[w];
             // the candidate's and row j's sub-block columns, from their owners' registers
-            const bool own_j = j >= r0 && j < r0 + nr;
+            const int oj = j % G; // the group owning row j
+            const bool own_j = g == oj;
             if (has_row && pos == bp)
             {
 #pragma unroll
@@ -232,41 +346,55 @@ __global__ void __launch_bounds__(256) panel_kernel(T *__restrict__ P, int lda, 
                     rowj_sub[i] = v[i];
             }
             __syncthreads();
-            // 2. publish candidate, its row, and the current row j if owned
-            if (tid == 0)
-            {
-                xstore(&X->cval[par][g], bv);
-                xstore_i(&X->cpos[par][g], bp);
-            }
-            if (bp != INT_MAX)
+            // 2. publish the candidate's row and row j (if owned); once those stores are
+            // complete, the tagged record that signals them
+            // (a candidate that is row j itself is not published twice: readers take row j)
+            if (bp != INT_MAX && bp != j)
                 for (int c = tid; c < jb; c += 256)
                     xstore(&X->crow[par][g][c],
-                           (c >= j0 && c < j0 + kb) ? cand_sub[c - j0] : L[(bp - r0) * LDP + c]);
+                           (c >= j0 && c < j0 + kb) ? cand_sub[c - j0] : L[(bp / G) * LDP + c]);
             if (own_j)
                 for (int c = tid; c < jb; c += 256)
-                    xstore(&X->rowj[par][c], (c >= j0 && c < j0 + kb) ? rowj_sub[c - j0] : L[(j - r0) * LDP + c]);
-            // 3. barrier
-            if (G > 1)
-                grid_barrier(&X->counter, base + (unsigned)G * (unsigned)(j + 1), &status[1]);
-            else
-                __syncthreads();
-            // 4. winner (identical in every group); meanwhile the second wave fetches row j, the
-            // pivot row whenever no interchange is due (the usual case), in the same round trip
-            if (tid >= 64 && tid < 64 + jb)
-                srowj[tid - 64] = xload(&X->rowj[par][tid - 64]);
+                    xstore(&X->rowj[par][c], (c >= j0 && c < j0 + kb) ? rowj_sub[c - j0] : L[(j / G) * LDP + c]);
+            __builtin_amdgcn_s_waitcnt(0);
+            __syncthreads();
+            if (tid == 0)
+            {
+                xstore_u64(&X->rec.slot[par][g].key,
+                           (bp == INT_MAX ? 0ull : (unsigned long long)__double_as_longlong(bv)) | ktag);
+                xstore_u32(&X->rec.slot[par][g].pos, (unsigned)bp | ptag);
+            }
+            // 3. every group's record of this round (wave 0, a group per lane) and the winner,
+            // identical in every group; wave 1 fetches row j as soon as its owner's record is
+            // in -- it is the pivot row whenever no interchange is due (the usual case)
             if (tid < 64)
             {
-                // candidates (|a| >= 0, or -1 with position INT_MAX for a group without one)
                 unsigned long long key = 0;
                 unsigned row = 0xFFFFFFFFu;
                 int bq = 0;
                 for (int q = tid; q < G; q += 64)
                 {
-                    const double qv = xload(&X->cval[par][q]);
-                    const int qp = xload_i(&X->cpos[par][q]);
-                    const unsigned long long kk = pivot_key(qv);
-                    if (qp != INT_MAX && qv >= 0.0 && (kk > key || (kk == key && (unsigned)qp < row)))
-                        key = kk, row = (unsigned)qp, bq = q;
+                    unsigned long long kw;
+                    unsigned pw;
+                    unsigned spins = 0;
+                    const unsigned long long t0 = wall_clock64();
+                    for (;;)
+                    {
+                        kw = xload_u64(&X->rec.slot[par][q].key);
+                        pw = xload_u32(&X->rec.slot[par][q].pos);
+                        if ((kw >> 63) == (ktag >> 63) && (pw >> 31) == (ptag >> 31))
+                            break;
+                        if (wait_expired(spins, t0, &status[1]))
+                        {
+                            xstore_i(&status[1], 1);
+                            break;
+                        }
+                        __builtin_amdgcn_s_sleep(1);
+                    }
+                    const unsigned long long kk = kw & ~(1ull << 63);
+                    const unsigned qp = pw & 0x7FFFFFFFu;
+                    if (qp != (unsigned)INT_MAX && (kk > key || (kk == key && qp < row)))
+                        key = kk, row = qp, bq = q;
                 }
                 unsigned long long km;
                 const unsigned wrow = wave_argmax(key, row, &km);
@@ -276,28 +404,48 @@ __global__ void __launch_bounds__(256) panel_kernel(T *__restrict__ P, int lda, 
                 if (tid == 0)
                     s_pos = wrow == 0xFFFFFFFFu ? INT_MAX : (int)wrow, s_grp = wgq;
             }
+            else if (tid < 128)
+            {
+                unsigned pw;
+                unsigned spins = 0;
+                const unsigned long long t0 = wall_clock64();
+                for (;;)
+                {
+                    pw = xload_u32(&X->rec.slot[par][oj].pos);
+                    if ((pw >> 31) == (ptag >> 31))
+                        break;
+                    if (wait_expired(spins, t0, &status[1]))
+                    {
+                        xstore_i(&status[1], 1);
+                        break;
+                    }
+                    __builtin_amdgcn_s_sleep(1);
+                }
+                if (tid - 64 < jb)
+                    srowj[tid - 64] = xload(&X->rowj[par][tid - 64]);
+            }
             __syncthreads();
             int p = s_pos;
             const int wg = s_grp;
             const bool none = (p == INT_MAX); // every candidate NaN: keep row j in place
             if (none)
                 p = j;
-            // the winner's published row equals row j when p == j (the same values published twice)
+            // when p == j the winner is row j (which its owner does not publish as a candidate)
             for (int c = tid; c < jb; c += 256)
                 prow[c] = (p == j) ? srowj[c] : xload(&X->crow[par][wg][c]);
             __syncthreads();
             // 5. interchange: LDS rows outside the sub-block, the owners' registers inside
-            const bool own_p = p >= r0 && p < r0 + nr;
+            const bool own_p = p % G == g;
             if (p != j)
             {
                 if (own_j)
                     for (int c = tid; c < jb; c += 256)
                         if (c < j0 || c >= j0 + kb)
-                            L[(j - r0) * LDP + c] = prow[c];
+                            L[(j / G) * LDP + c] = prow[c];
                 if (own_p)
                     for (int c = tid; c < jb; c += 256)
                         if (c < j0 || c >= j0 + kb)
-                            L[(p - r0) * LDP + c] = srowj[c];
+                            L[(p / G) * LDP + c] = srowj[c];
                 if (has_row && pos == j)
                 {
 #pragma unroll
@@ -366,38 +514,30 @@ __global__ void __launch_bounds__(256) panel_kernel(T *__restrict__ P, int lda, 
             }
         }
         __syncthreads();
-        // 8. the columns right of the sub-block: rows below its pivots take its kb updates
-        // (multipliers in registers), its pivot rows their U12 rows
-        if (has_row && j0 + kb < jb)
+        // 8. the columns right of the sub-block: rows below its pivots take its kb updates,
+        // its pivot rows their U12 rows
+        if (j0 + kb < jb)
         {
-            if (pos >= j0 + kb)
+            const int c0 = j0 + kb, ncols = jb - c0;
+            // local rows: [rp, rb) hold the sub-block's pivot positions, [rb, nr) the rows below
+            const int rp = j0 - g > 0 ? (j0 - g + G - 1) / G : 0;
+            const int rb = c0 - g > 0 ? (c0 - g + G - 1) / G : 0;
+            const int nct = (ncols + 7) / 8;
+            if (rb < nr)
             {
-                // eight columns at a time: eight independent FMA chains (one wave per SIMD
-                // here, so a single chain would wait on every FMA's latency)
-                for (int cb = j0 + kb; cb < jb; cb += 8)
-                {
-                    T x[8];
-#pragma unroll
-                    for (int q = 0; q < 8; ++q)
-                        x[q] = (cb + q < jb) ? L[r * LDP + cb + q] : T(0);
-#pragma unroll
-                    for (int i = 0; i < KB; ++i)
-                        if (i < kb && !((zm >> i) & 1u))
-                        {
-#pragma unroll
-                            for (int q = 0; q < 8; ++q)
-                                x[q] = fma(-v[i], ucur[i][min(cb + q, jb - 1)], x[q]);
-                        }
-#pragma unroll
-                    for (int q = 0; q < 8; ++q)
-                        if (cb + q < jb)
-                            L[r * LDP + cb + q] = x[q];
-                }
+                if (nct >= 5)
+                    subblock_update<T, 8, KB, LDP, NB>(L, ucur, rb, nr, R, j0, kb, c0, jb, zm);
+                else if (nct >= 3)
+                    subblock_update<T, 4, KB, LDP, NB>(L, ucur, rb, nr, R, j0, kb, c0, jb, zm);
+                else if (nct == 2)
+                    subblock_update<T, 2, KB, LDP, NB>(L, ucur, rb, nr, R, j0, kb, c0, jb, zm);
+                else
+                    subblock_update<T, 1, KB, LDP, NB>(L, ucur, rb, nr, R, j0, kb, c0, jb, zm);
             }
-            else if (pos >= j0)
+            for (int idx = tid; idx < (min(rb, nr) - rp) * ncols; idx += 256)
             {
-                for (int c = j0 + kb; c < jb; ++c)
-                    L[r * LDP + c] = ucur[pos - j0][c];
+                const int r = rp + idx / ncols, c = c0 + idx % ncols;
+                L[r * LDP + c] = ucur[g + r * G - j0][c];
             }
         }
         __syncthreads();
@@ -412,7 +552,7 @@ __global__ void __launch_bounds__(256) panel_kernel(T *__restrict__ P, int lda, 
     for (int idx = tid; idx < nr * jb; idx += 256)
     {
         const int r = idx / jb, c = idx - r * jb;
-        P[(size_t)(r0 + r) * lda + c] = L[r * LDP + c];
+        P[(size_t)(g + r * G) * lda + c] = L[r * LDP + c];
     }
 }
 
@@ -1578,12 +1718,18 @@ Workspace g_ws;
 volatile int *g_status_host = nullptr;
 int *g_status_host_dev = nullptr;
 
-__global__ void factor_reset_kernel(unsigned *__restrict__ x0, int *__restrict__ status)
+__global__ void factor_reset_kernel(PanelRec *__restrict__ rec, int *__restrict__ status)
 {
     main_stream_priority();
+    // records tagged 1: never taken for rounds 0 and 1 (tag 0), and overwritten before the
+    // rounds with tag 1 (2 and 3) read them
+    for (int i = threadIdx.x; i < 2 * kMaxGroups; i += blockDim.x)
+    {
+        (&rec->slot[0][0])[i].key = 1ull << 63;
+        (&rec->slot[0][0])[i].pos = 1u << 31;
+    }
     if (threadIdx.x == 0)
     {
-        *x0 = 0u;             // panel exchange sequence
         status[0] = 0;        // first zero pivot
         status[1] = 0;        // a bounded wait timed out
         status[2] = 0x7F7F7F7F; // no-pivot zero pivots: min
@@ -1668,7 +1814,8 @@ template <typename T> int factor_impl(int n, T *A, int *piv, int *info_out, bool
         return st;
     static bool attr_set[2] = {false, false};
     const int rmax = panel_rows_max<T>();
-    const size_t lds = sizeof(T) * (size_t)rmax * (NB + 1);
+    // one row beyond the group's rows: a sink for the sub-block update's out-of-range stores
+    const size_t lds = sizeof(T) * (size_t)(rmax + 1) * (NB + 1);
     if (!attr_set[sizeof(T) == 8])
     {
         PVL_HIP(hipFuncSetAttribute((const void *)panel_kernel<T>, hipFuncAttributeMaxDynamicSharedMemorySize,
@@ -1677,7 +1824,7 @@ template <typename T> int factor_impl(int n, T *A, int *piv, int *info_out, bool
     }
     PanelX<T> *X = (PanelX<T> *)g_ws.panelx;
     int *status = g_ws.err;
-    factor_reset_kernel<<<1, 64, 0, stream()>>>((unsigned *)X, status);
+    factor_reset_kernel<<<1, 256, 0, stream()>>>(&X->rec, status);
     PVL_HIP(hipGetLastError());
     unsigned base = 0;
     // Panel width: 32 up to n = 2048, where single-workgroup register panels cover every
@@ -1757,9 +1904,8 @@ template <typename T> int factor_impl(int n, T *A, int *piv, int *info_out, bool
             int mm = m, jbb = jb, RR = R;
             unsigned bb = base;
             void *args[] = {&P, &lda, &mm, &jbb, &RR, &row_base, &piv, &X, &bb, &status};
-            if (G > 1)
-                base += (unsigned)G * (unsigned)jb;
-            const size_t lds_now = sizeof(T) * (size_t)R * (NB + 1);
+            base += (unsigned)jb; // one exchange round per column, consecutive over the launches
+            const size_t lds_now = sizeof(T) * (size_t)(R + 1) * (NB + 1);
             {
                 ProfScope prof(sizeof(T) == 8 ? "lu_panel_f64" : "lu_panel_f32");
                 if (int st = launch_grid_synced((const void *)panel_kernel<T>, G, 256, args, lds_now, stream()))
