@@ -251,7 +251,7 @@ __global__ void __launch_bounds__(256) panel_kernel(T *__restrict__ P, int lda, 
     extern __shared__ __align__(16) unsigned char smem_raw[];
     constexpr int LDP = NB + 1;
     T *L = (T *)smem_raw;
-    __shared__ T prow[NB], srowj[NB];
+    __shared__ T prow[NB], srowj[NB], s_invj;
     __shared__ double red_v[4];
     __shared__ int red_p[4];
     __shared__ int s_pos, s_grp;
@@ -282,7 +282,7 @@ __global__ void __launch_bounds__(256) panel_kernel(T *__restrict__ P, int lda, 
     // sub-block): every consumer (the interchange, U12) treats them the same way.
     constexpr int KB = 8;
     __shared__ T ucur[KB][NB]; // the sub-block's pivot rows as published, then U12 in place
-    __shared__ T cand_sub[KB], rowj_sub[KB];
+    __shared__ T wsub[4][KB], rowj_sub[KB]; // each wave's candidate's sub-block columns; row j's
     const int r = tid; // this thread's row (R <= 256 rows per group)
     const bool has_row = r < nr;
     const int pos = g + r * G;
@@ -305,7 +305,11 @@ __global__ void __launch_bounds__(256) panel_kernel(T *__restrict__ P, int lda, 
             const unsigned tagbit = (rnd >> 1) & 1u;
             const unsigned long long ktag = (unsigned long long)tagbit << 63;
             const unsigned ptag = tagbit << 31;
-            // 1. local candidate: per lane, then over the wave (DPP moves), then over the waves
+            // 1. local candidate: per lane, then over the wave (DPP moves), then over the waves;
+            // each wave's winner stages its sub-block columns beside the wave's record, and the
+            // owner of row j its own, before the one barrier
+            const int oj = j % G; // the group owning row j
+            const bool own_j = g == oj;
             {
                 unsigned long long key = 0;
                 unsigned row = 0xFFFFFFFFu;
@@ -323,21 +327,12 @@ __global__ void __launch_bounds__(256) panel_kernel(T *__restrict__ P, int lda, 
                     red_v[tid >> 6] = none ? -1.0 : __longlong_as_double((long long)km);
                     red_p[tid >> 6] = none ? INT_MAX : (int)wrow;
                 }
-            }
-            __syncthreads();
-            double bv = red_v[0];
-            int bp = red_p[0];
-            for (int w = 1; w < 4; ++w)
-                if (better(red_v[w], red_p[w], bv, bp))
-                    bv = red_v[w], bp = red_p[w];
-            // the candidate's and row j's sub-block columns, from their owners' registers
-            const int oj = j % G; // the group owning row j
-            const bool own_j = g == oj;
-            if (has_row && pos == bp)
-            {
+                if (has_row && (unsigned)pos == wrow)
+                {
 #pragma unroll
-                for (int i = 0; i < KB; ++i)
-                    cand_sub[i] = v[i];
+                    for (int i = 0; i < KB; ++i)
+                        wsub[tid >> 6][i] = v[i];
+                }
             }
             if (has_row && pos == j)
             {
@@ -346,13 +341,18 @@ __global__ void __launch_bounds__(256) panel_kernel(T *__restrict__ P, int lda, 
                     rowj_sub[i] = v[i];
             }
             __syncthreads();
+            double bv = red_v[0];
+            int bp = red_p[0], bw = 0;
+            for (int w = 1; w < 4; ++w)
+                if (better(red_v[w], red_p[w], bv, bp))
+                    bv = red_v[w], bp = red_p[w], bw = w;
             // 2. publish the candidate's row and row j (if owned); once those stores are
             // complete, the tagged record that signals them
             // (a candidate that is row j itself is not published twice: readers take row j)
             if (bp != INT_MAX && bp != j)
                 for (int c = tid; c < jb; c += 256)
                     xstore(&X->crow[par][g][c],
-                           (c >= j0 && c < j0 + kb) ? cand_sub[c - j0] : L[(bp / G) * LDP + c]);
+                           (c >= j0 && c < j0 + kb) ? wsub[bw][c - j0] : L[(bp / G) * LDP + c]);
             if (own_j)
                 for (int c = tid; c < jb; c += 256)
                     xstore(&X->rowj[par][c], (c >= j0 && c < j0 + kb) ? rowj_sub[c - j0] : L[(j / G) * LDP + c]);
@@ -422,7 +422,14 @@ __global__ void __launch_bounds__(256) panel_kernel(T *__restrict__ P, int lda, 
                     __builtin_amdgcn_s_sleep(1);
                 }
                 if (tid - 64 < jb)
-                    srowj[tid - 64] = xload(&X->rowj[par][tid - 64]);
+                {
+                    const T rj = xload(&X->rowj[par][tid - 64]);
+                    srowj[tid - 64] = rj;
+                    // the reciprocal of row j's diagonal, ready for the usual case that row j
+                    // is the pivot row (the division runs while wave 0 still polls)
+                    if (tid - 64 == j)
+                        s_invj = rj != T(0) ? T(1) / rj : T(0);
+                }
             }
             __syncthreads();
             int p = s_pos;
@@ -430,10 +437,16 @@ __global__ void __launch_bounds__(256) panel_kernel(T *__restrict__ P, int lda, 
             const bool none = (p == INT_MAX); // every candidate NaN: keep row j in place
             if (none)
                 p = j;
-            // when p == j the winner is row j (which its owner does not publish as a candidate)
-            for (int c = tid; c < jb; c += 256)
-                prow[c] = (p == j) ? srowj[c] : xload(&X->crow[par][wg][c]);
-            __syncthreads();
+            // the pivot row: row j itself when p == j (its owner does not publish it as a
+            // candidate), else the winner's published row
+            const T *prow_p = srowj;
+            if (p != j)
+            {
+                for (int c = tid; c < jb; c += 256)
+                    prow[c] = xload(&X->crow[par][wg][c]);
+                __syncthreads();
+                prow_p = prow;
+            }
             // 5. interchange: LDS rows outside the sub-block, the owners' registers inside
             const bool own_p = p % G == g;
             if (p != j)
@@ -441,7 +454,7 @@ __global__ void __launch_bounds__(256) panel_kernel(T *__restrict__ P, int lda, 
                 if (own_j)
                     for (int c = tid; c < jb; c += 256)
                         if (c < j0 || c >= j0 + kb)
-                            L[(j / G) * LDP + c] = prow[c];
+                            L[(j / G) * LDP + c] = prow_p[c];
                 if (own_p)
                     for (int c = tid; c < jb; c += 256)
                         if (c < j0 || c >= j0 + kb)
@@ -450,7 +463,7 @@ __global__ void __launch_bounds__(256) panel_kernel(T *__restrict__ P, int lda, 
                 {
 #pragma unroll
                     for (int i = 0; i < KB; ++i)
-                        v[i] = (i < kb) ? prow[j0 + i] : v[i];
+                        v[i] = (i < kb) ? prow_p[j0 + i] : v[i];
                 }
                 else if (has_row && pos == p)
                 {
@@ -460,14 +473,14 @@ __global__ void __launch_bounds__(256) panel_kernel(T *__restrict__ P, int lda, 
                 }
             }
             for (int c = tid; c < jb; c += 256)
-                ucur[t][c] = prow[c];
+                ucur[t][c] = prow_p[c];
             if (tid == 0)
                 spiv[j] = row_base + p; // global store deferred: see the end of the kernel
             // 6. scale by the reciprocal pivot and update the sub-block's columns (registers)
-            const T pv = prow[j];
+            const T pv = prow_p[j];
             if (pv != T(0))
             {
-                const T inv = T(1) / pv;
+                const T inv = p == j ? s_invj : T(1) / pv;
                 if (has_row && pos > j)
                 {
                     const T l = v[t] * inv;
@@ -475,7 +488,7 @@ __global__ void __launch_bounds__(256) panel_kernel(T *__restrict__ P, int lda, 
 #pragma unroll
                     for (int t2 = t + 1; t2 < KB; ++t2)
                         if (t2 < kb)
-                            v[t2] = fma(-l, prow[j0 + t2], v[t2]);
+                            v[t2] = fma(-l, prow_p[j0 + t2], v[t2]);
                 }
             }
             else
@@ -1187,6 +1200,47 @@ __device__ __forceinline__ void block_product(const T (*M)[BS + 1], const T (*X)
     }
 }
 
+// Substitution with the diagonal block of one block row (LAPACK dtrsv order): lane = row,
+// NQ right-hand sides per wave in registers; row k's value is broadcast with readlane.  The
+// lane's row of the block is read from LDS eight columns at a time, the next eight while the
+// current ones are used, so a step waits on no LDS access: the chain per step is the FMA,
+// the broadcast and (upper) the division on the pivot lane.
+template <typename T, bool LOWER, int BS, int NQ, int QW>
+__device__ __forceinline__ void flow_substitute(T (&x)[QW], const T (*Ls)[BS + 1], int lane, T diag)
+{
+    constexpr int KC = 8;
+    T lc[KC], ln[KC];
+#pragma unroll
+    for (int u = 0; u < KC; ++u)
+        lc[u] = Ls[lane][LOWER ? u : BS - 1 - u];
+    for (int kc = 0; kc < BS; kc += KC)
+    {
+        if (kc + KC < BS)
+        {
+#pragma unroll
+            for (int u = 0; u < KC; ++u)
+                ln[u] = Ls[lane][LOWER ? kc + KC + u : BS - 1 - (kc + KC + u)];
+        }
+#pragma unroll
+        for (int u = 0; u < KC; ++u)
+        {
+            const int k = LOWER ? kc + u : BS - 1 - (kc + u);
+#pragma unroll
+            for (int j = 0; j < NQ; ++j)
+            {
+                if (!LOWER && lane == k)
+                    x[j] = x[j] / diag;
+                const T xk = bcast_lane(x[j], k);
+                if (LOWER ? lane > k : lane < k)
+                    x[j] = fma(-lc[u], xk, x[j]);
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < KC; ++u)
+            lc[u] = ln[u];
+    }
+}
+
 template <typename T, bool LOWER, int BS, int NT>
 __global__ void __launch_bounds__(NT) trsv_flow_kernel(const T *__restrict__ A, int lda, int n, T *__restrict__ B,
                                                         int nrhs, const float *__restrict__ dinv,
@@ -1297,8 +1351,7 @@ __global__ void __launch_bounds__(NT) trsv_flow_kernel(const T *__restrict__ A, 
     else
     {
         // wave wv holds the right-hand sides wv, wv + 4, ... (at most kMaxRhs / 4) in
-        // registers and advances them together; the current row's value is broadcast with
-        // readlane
+        // registers and advances them together (flow_substitute).
         static_assert(BS == 64 && NT == 256, "FP64 substitution: one row per lane, four waves");
         const int lane = tid & 63, wv = tid >> 6;
         constexpr int QW = kMaxRhs / 4;
@@ -1309,22 +1362,21 @@ __global__ void __launch_bounds__(NT) trsv_flow_kernel(const T *__restrict__ A, 
             const int q = wv + 4 * j;
             x[j] = q < nrhs ? acc[lane][q] : T(0);
         }
+        const int nq = (nrhs - wv + 3) / 4; // right-hand sides of this wave (uniform)
         const T lrow_diag = Ls[lane][lane];
-        for (int kk = 0; kk < BS; ++kk)
+        // the count of right-hand sides is a template argument: a runtime bound inside the
+        // unrolled steps turns into a branch cascade that copies every x at every exit
+        switch (nq)
         {
-            const int k = LOWER ? kk : BS - 1 - kk;
-            const T lk = Ls[lane][k];
-#pragma unroll
-            for (int j = 0; j < QW; ++j)
-            {
-                if (wv + 4 * j >= nrhs) // uniform
-                    break;
-                if (!LOWER && lane == k)
-                    x[j] = x[j] / lrow_diag;
-                const T xk = bcast_lane(x[j], k);
-                if (LOWER ? lane > k : lane < k)
-                    x[j] = fma(-lk, xk, x[j]);
-            }
+        case 1: flow_substitute<T, LOWER, BS, 1>(x, Ls, lane, lrow_diag); break;
+        case 2: flow_substitute<T, LOWER, BS, 2>(x, Ls, lane, lrow_diag); break;
+        case 3: flow_substitute<T, LOWER, BS, 3>(x, Ls, lane, lrow_diag); break;
+        case 4: flow_substitute<T, LOWER, BS, 4>(x, Ls, lane, lrow_diag); break;
+        case 5: flow_substitute<T, LOWER, BS, 5>(x, Ls, lane, lrow_diag); break;
+        case 6: flow_substitute<T, LOWER, BS, 6>(x, Ls, lane, lrow_diag); break;
+        case 7: flow_substitute<T, LOWER, BS, 7>(x, Ls, lane, lrow_diag); break;
+        case 8: flow_substitute<T, LOWER, BS, 8>(x, Ls, lane, lrow_diag); break;
+        default: break;
         }
 #pragma unroll
         for (int j = 0; j < QW; ++j)
