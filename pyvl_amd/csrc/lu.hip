@@ -1178,25 +1178,35 @@ __global__ void flow_reset_kernel(unsigned *__restrict__ sync, int *__restrict__
 constexpr int kXs = kMaxRhs + 4; // row stride of the right-hand-side tiles (16-byte rows)
 constexpr int kFlowBs32 = 128;   // block rows of the FP32 solves (and of their diagonal inverses)
 
-// out(r, q..q+3) = sum_c M[r][c] X[c][q..q+3] for the BS rows and nq4 groups of 4 columns,
-// each sum in column order c = 0, 1, ... (one thread per row and column group)
+// out(r, q, s) for s = sum_c M[r][c] X[c][q], the BS x (columns < 4 nq4) block product on
+// MFMA: wave w takes rows 16w .. 16w + 15 and every 16-column tile, K in steps of 4 from
+// c = 0 (the order within a step is the matrix core's).  X columns at or beyond 4 nq4 enter
+// as zeros and their outputs are not reported.
 template <typename T, int BS, int NT, typename F>
-__device__ __forceinline__ void block_product(const T (*M)[BS + 1], const T (*X)[kXs], int nq4, F &&out)
+__device__ __forceinline__ void block_product_mfma(const T (*M)[BS + 1], const T (*X)[kXs], int nq4, F &&out)
 {
-    for (int it = threadIdx.x; it < BS * nq4; it += NT)
+    static_assert(BS == 16 * (NT / 64), "one 16-row tile per wave");
+    using acc_t = typename Mfma<T>::acc_t;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int lr = lane & 15, lk = lane >> 4;
+    const int ncol = 4 * nq4, nct = (ncol + 15) / 16;
+    for (int ct = 0; ct < nct; ++ct)
     {
-        const int r = it / nq4, q = 4 * (it - r * nq4);
-        T s0 = T(0), s1 = T(0), s2 = T(0), s3 = T(0);
-#pragma unroll 16
-        for (int c = 0; c < BS; ++c)
+        const int q = 16 * ct + lr;
+        acc_t c = acc_t{0, 0, 0, 0};
+#pragma unroll 8
+        for (int kk = 0; kk < BS; kk += 4)
         {
-            const T m = M[r][c];
-            s0 = fma(m, X[c][q], s0);
-            s1 = fma(m, X[c][q + 1], s1);
-            s2 = fma(m, X[c][q + 2], s2);
-            s3 = fma(m, X[c][q + 3], s3);
+            const T a = M[16 * w + lr][kk + lk];
+            const T b = q < ncol ? X[kk + lk][q] : T(0);
+            c = Mfma<T>::mma(a, b, c);
         }
-        out(r, q, s0, s1, s2, s3);
+        if (q < ncol)
+        {
+#pragma unroll
+            for (int v = 0; v < 4; ++v)
+                out(16 * w + Mfma<T>::out_row(lk, v), q, c[v]);
+        }
     }
 }
 
@@ -1320,12 +1330,7 @@ __global__ void __launch_bounds__(NT) trsv_flow_kernel(const T *__restrict__ A, 
             xs[r][q] = (r < nc && q < nrhs) ? xload(&B[(size_t)(c0 + r) * nrhs + q]) : T(0);
         }
         __syncthreads();
-        block_product<T, BS, NT>(Ls, xs, nq4, [&](int r, int q, T a0, T a1, T a2, T a3) {
-            acc[r][q] -= a0;
-            acc[r][q + 1] -= a1;
-            acc[r][q + 2] -= a2;
-            acc[r][q + 3] -= a3;
-        });
+        block_product_mfma<T, BS, NT>(Ls, xs, nq4, [&](int r, int q, T a) { acc[r][q] -= a; });
         __syncthreads();
         if (k + 1 < t)
         {
@@ -1339,13 +1344,9 @@ __global__ void __launch_bounds__(NT) trsv_flow_kernel(const T *__restrict__ A, 
     const T poison = s_ok ? T(0) : T(NAN);
     if constexpr (sizeof(T) == 4)
     {
-        block_product<T, BS, NT>(Ls, acc, nq4, [&](int r, int q, T a0, T a1, T a2, T a3) {
-            if (r < nr)
-            {
-                const T v[4] = {a0, a1, a2, a3};
-                for (int j = 0; j < 4 && q + j < nrhs; ++j)
-                    xstore(&B[(size_t)(r0 + r) * nrhs + q + j], v[j] + poison);
-            }
+        block_product_mfma<T, BS, NT>(Ls, acc, nq4, [&](int r, int q, T a) {
+            if (r < nr && q < nrhs)
+                xstore(&B[(size_t)(r0 + r) * nrhs + q], a + poison);
         });
     }
     else
