@@ -178,19 +178,20 @@ __global__ void __launch_bounds__(kBlock)
 }
 
 // Normal-reduced assembly (induction_matrix3), out[n_cpts][n_surf]: one lane per control
-// point, one wave per run of kAsmSurfaces consecutive surfaces executing the run's program
-// (build_assembly_program): each line of the run is evaluated once, as n . K, and kept in an
-// LDS slot while a later surface needs it; a surface's value is the sum of its lines' +-n.K
-// in CSR order (src/core/flow_solver.c:188-202 sums the vectors first and then dots with the
-// normal: the same terms, another association).  Program entries are wave-uniform (scalar
-// loads and branches); per lane the inner step is one K and one LDS access at most.  The run's
-// results are staged in LDS and stored as rows of kAsmSurfaces consecutive doubles.
+// point, one wave per run of up to kAsmGroups groups of kAsmSurfaces consecutive surfaces,
+// executing the run's program (build_assembly_program): each line of the run is evaluated
+// once, as n . K, and kept in an LDS slot while a later surface of the run needs it; a
+// surface's value is the sum of its lines' +-n.K in CSR order (src/core/flow_solver.c:188-202
+// sums the vectors first and then dots with the normal: the same terms, another association).
+// Program entries are wave-uniform (scalar loads and branches); per lane the inner step is one
+// K and one LDS access at most.  Each group's results are staged in LDS and stored as rows of
+// kAsmSurfaces consecutive doubles.
 template <int PREC>
 __global__ void __launch_bounds__(kBlock)
-    assembly_kernel(unsigned n_cpts, unsigned n_surf, const uint32_t *__restrict__ run_off,
-                    const uint32_t *__restrict__ prog, const LineGeom *__restrict__ geom,
-                    const double *__restrict__ cpts, const double *__restrict__ normals, double *__restrict__ out,
-                    double tol)
+    assembly_kernel(unsigned n_cpts, unsigned n_surf, unsigned n_runs, const uint32_t *__restrict__ run_off,
+                    const uint32_t *__restrict__ run_grp, const uint32_t *__restrict__ prog,
+                    const LineGeom *__restrict__ geom, const double *__restrict__ cpts,
+                    const double *__restrict__ normals, double *__restrict__ out, double tol)
 {
     main_stream_priority();
     __shared__ double slots[4][kAsmSlots][kWave];
@@ -203,17 +204,19 @@ __global__ void __launch_bounds__(kBlock)
     const double nx = normals[3 * cpc + 0], ny = normals[3 * cpc + 1], nz = normals[3 * cpc + 2];
     const double tol2 = tol * tol;
     const unsigned run = blockIdx.x * 4 + w;
-    const unsigned s0 = run * kAsmSurfaces;
-    if (s0 >= n_surf)
+    if (run >= n_runs)
         return;
     const uint32_t k0 = run_off[run], k1 = run_off[run + 1];
+    unsigned gi = 0;
+    uint32_t grp = run_grp[run * kAsmGroups];
+    unsigned ns = min(kAsmSurfaces, n_surf - grp * kAsmSurfaces); // surfaces of the current group
     unsigned sl = 0;
     double acc = 0.0;
     // pairs of entries: both entries' scalar loads (program words and line geometry) are
     // issued before either is evaluated, so one wait covers two evaluations (scalar loads
     // return out of order: a wave can only wait for all of them)
     auto step = [&](uint32_t e, uint32_t op, const LineGeom &g) {
-        const uint32_t kind = op & 3u, slot = (op >> 2) & 7u;
+        const uint32_t kind = op & 3u, slot = (op >> 2) & 15u;
         double v;
         if (kind == kAsmLoad)
             v = slots[w][slot][lane];
@@ -233,8 +236,8 @@ __global__ void __launch_bounds__(kBlock)
             ++sl;
         }
     };
-    uint32_t k = k0;
-    for (; k + 1 < k1; k += 2)
+    // groups have even entry counts (padded), so a group ends on a pair boundary
+    for (uint32_t k = k0; k < k1; k += 2)
     {
         // a kAsmNone entry names line 0 (the geometry buffer always has one element)
         const uint32_t ea = prog[2 * k], opa = prog[2 * k + 1], eb = prog[2 * k + 2], opb = prog[2 * k + 3];
@@ -245,21 +248,24 @@ __global__ void __launch_bounds__(kBlock)
                      "s"(gb.len));
         step(ea, opa, ga);
         step(eb, opb, gb);
-    }
-    if (k < k1)
-    {
-        const uint32_t ea = prog[2 * k], opa = prog[2 * k + 1];
-        const LineGeom ga = geom[ea & kInvalidId];
-        step(ea, opa, ga);
-    }
-    // rows of the run: lane = 8 rows x 8 surfaces per store instruction
-    const unsigned ns = (n_surf - s0) < kAsmSurfaces ? (n_surf - s0) : kAsmSurfaces;
-    const unsigned c = lane % kAsmSurfaces;
-    for (unsigned r = lane / kAsmSurfaces; r < kCptTile; r += kWave / kAsmSurfaces)
-    {
-        const unsigned p = cp0 + r;
-        if (p < n_cpts && c < ns)
-            out[(size_t)p * n_surf + s0 + c] = res[w][c][r];
+        if (sl == ns)
+        {
+            // the group's rows: lane = 8 rows x 8 surfaces per store instruction
+            const unsigned s0 = grp * kAsmSurfaces, c = lane % kAsmSurfaces;
+            for (unsigned r = lane / kAsmSurfaces; r < kCptTile; r += kWave / kAsmSurfaces)
+            {
+                const unsigned p = cp0 + r;
+                if (p < n_cpts && c < ns)
+                    out[(size_t)p * n_surf + s0 + c] = res[w][c][r];
+            }
+            sl = 0;
+            if (++gi < kAsmGroups)
+            {
+                grp = run_grp[run * kAsmGroups + gi];
+                if (grp != kAsmNoGroup)
+                    ns = min(kAsmSurfaces, n_surf - grp * kAsmSurfaces);
+            }
+        }
     }
 }
 
@@ -556,8 +562,9 @@ int launch_surface(const pvl_mesh *m, double tol, const double *pos, unsigned n_
     LineGeom *g;
     if (int st = compute_line_geom(m, pos, tol, &g))
         return st;
-    constexpr int spw = NORMAL ? kAsmSurfaces : 8;
-    const dim3 grid((m->n_surfaces + 4 * spw - 1) / (4 * spw), (n_cpts + kCptTile - 1) / kCptTile);
+    constexpr int spw = 8;
+    const dim3 grid(NORMAL ? (m->n_asm_runs + 3) / 4 : (m->n_surfaces + 4 * spw - 1) / (4 * spw),
+                    (n_cpts + kCptTile - 1) / kCptTile);
     if (grid.y > 65535u)
     {
         set_error("too many control points (%u) for one launch", n_cpts);
@@ -567,7 +574,8 @@ int launch_surface(const pvl_mesh *m, double tol, const double *pos, unsigned n_
     dispatch_precision(precision, [&](auto P) {
         if constexpr (NORMAL)
             assembly_kernel<decltype(P)::value><<<grid, kBlock, 0, stream()>>>(
-                n_cpts, m->n_surfaces, m->d_asm_off, m->d_asm, g, cpts, normals, out, tol);
+                n_cpts, m->n_surfaces, m->n_asm_runs, m->d_asm_off, m->d_asm_grp, m->d_asm, g, cpts, normals, out,
+                tol);
         else
             surface_induction_kernel<NORMAL, decltype(P)::value, spw><<<grid, kBlock, 0, stream()>>>(
                 n_cpts, m->n_surfaces, m->d_offsets, m->d_entries, g, cpts, normals, out, tol);

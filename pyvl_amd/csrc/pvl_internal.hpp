@@ -205,23 +205,33 @@ struct pvl_mesh
     // line -> (surface, sign) transpose, for line circulations
     uint32_t *d_line_offsets = nullptr; // [n_lines+1]
     uint32_t *d_line_surfs = nullptr;   // [offsets[n_surfaces]] surface | kReversed
-    // assembly program (pvl::build_assembly_program): per run of kAsmSurfaces surfaces
+    // assembly program (pvl::build_assembly_program): per run of up to kAsmGroups groups
     uint32_t *d_asm_off = nullptr; // [n_runs + 1] entry offsets
+    uint32_t *d_asm_grp = nullptr; // [n_runs * kAsmGroups] groups of the run (kAsmNoGroup: none)
     uint32_t *d_asm = nullptr;     // [2 * entries] (line | kReversed, op)
+    unsigned n_asm_runs = 0;
 };
 
 namespace pvl
 {
-// The assembly (induction_matrix3) walks its surfaces in runs of kAsmSurfaces consecutive
-// surfaces, one run per wave.  Inside a run each line's n . K is evaluated once and kept in
-// one of kAsmSlots per-lane LDS slots while a later surface of the run still needs it.  The
-// program lists, per run and in CSR order, one entry per surface line: word 0 the line with
+// The assembly (induction_matrix3) walks its surfaces in groups of kAsmSurfaces consecutive
+// surfaces (group g: surfaces 8g .. 8g + 7) and its groups in runs of up to kAsmGroups, one
+// run per wave; a run's groups are chosen by shared lines (on a structured body: the same
+// columns of kAsmGroups neighbouring rows), so that most lines of a run serve two of its
+// surfaces.  Inside a run each line's n . K is evaluated once and kept in one of kAsmSlots
+// per-lane LDS slots while a later surface of the run still needs it.  The program lists,
+// per run, group by group and in CSR order, one entry per surface line: word 0 the line with
 // the orientation bit, word 1 the operation (kAsmEval / kAsmEvalStore / kAsmLoad / kAsmNone,
-// the slot in bits 2-4, kAsmEnd on a surface's last entry; a surface without lines gets one
-// kAsmNone entry).  Built on the host once per mesh, in O(entries).
+// the slot in bits 2-5, kAsmEnd on a surface's last entry; a surface without lines gets one
+// kAsmNone entry with kAsmEnd).  A group with an odd entry count is padded with one kAsmNone
+// entry without kAsmEnd (adds zero), so that groups start on even entries (the kernel takes
+// entries in pairs).  Built on the host once per mesh, in O(entries).
 constexpr unsigned kAsmSurfaces = 8;
-constexpr unsigned kAsmSlots = 4;
+constexpr unsigned kAsmGroups = 4;
+constexpr unsigned kAsmSlots = 10;
 constexpr uint32_t kAsmEval = 0, kAsmEvalStore = 1, kAsmLoad = 2, kAsmNone = 3, kAsmEnd = 1u << 8;
+constexpr uint32_t kAsmNoGroup = 0xFFFFFFFFu;
 void build_assembly_program(unsigned n_surfaces, const uint32_t *offsets, const uint32_t *entries,
-                            std::vector<uint32_t> &run_off, std::vector<uint32_t> &prog);
+                            std::vector<uint32_t> &run_off, std::vector<uint32_t> &run_grp,
+                            std::vector<uint32_t> &prog);
 } // namespace pvl

@@ -3,6 +3,7 @@
 // the whole library is one toolchain).
 #include "pvl_internal.hpp"
 
+#include <algorithm>
 #include <unordered_map>
 #include <vector>
 
@@ -198,84 +199,156 @@ int pvl_wake_topology(unsigned n_strips, unsigned line_rows, unsigned *n_lines_o
 // CSR order; a line used again later in the run is kept in an LDS slot, slots evicted by
 // farthest next use (a line whose slot was taken is evaluated again).
 void pvl::build_assembly_program(unsigned n_surfaces, const uint32_t *offsets, const uint32_t *entries,
-                                 std::vector<uint32_t> &run_off, std::vector<uint32_t> &prog)
+                                 std::vector<uint32_t> &run_off, std::vector<uint32_t> &run_grp,
+                                 std::vector<uint32_t> &prog)
 {
-    const unsigned n_runs = (n_surfaces + kAsmSurfaces - 1) / kAsmSurfaces;
+    constexpr uint32_t kNever = 0xFFFFFFFFu;
+    const unsigned n_groups = (n_surfaces + kAsmSurfaces - 1) / kAsmSurfaces;
+    auto g_k0 = [&](unsigned g) { return offsets[g * kAsmSurfaces]; };
+    auto g_k1 = [&](unsigned g) { return offsets[std::min(n_surfaces, (g + 1) * kAsmSurfaces)]; };
+    // line -> the (at most two, a line may repeat) groups whose surfaces use it
+    std::unordered_map<uint32_t, std::pair<uint32_t, uint32_t>> line_groups;
+    line_groups.reserve(offsets[n_surfaces]);
+    for (unsigned g = 0; g < n_groups; ++g)
+        for (uint32_t k = g_k0(g); k < g_k1(g); ++k)
+        {
+            auto it = line_groups.emplace(entries[k] & kInvalidId, std::make_pair(g, kNever)).first;
+            if (it->second.first != g && it->second.second == kNever)
+                it->second.second = g;
+        }
+    // runs: from the first unassigned group, repeatedly the unassigned group sharing the most
+    // lines with the run's last group (ties: the lowest index), up to kAsmGroups
+    std::vector<char> assigned(n_groups, 0);
+    std::vector<uint32_t> runs; // kAsmGroups entries per run
+    std::unordered_map<uint32_t, unsigned> shared;
+    for (unsigned g0 = 0; g0 < n_groups; ++g0)
+    {
+        if (assigned[g0])
+            continue;
+        uint32_t run[kAsmGroups];
+        unsigned nr = 0;
+        run[nr++] = g0;
+        assigned[g0] = 1;
+        while (nr < kAsmGroups)
+        {
+            shared.clear();
+            const uint32_t last = run[nr - 1];
+            for (uint32_t k = g_k0(last); k < g_k1(last); ++k)
+            {
+                const auto &lg = line_groups[entries[k] & kInvalidId];
+                for (uint32_t g2 : {lg.first, lg.second})
+                    if (g2 != kNever && !assigned[g2])
+                        ++shared[g2];
+            }
+            uint32_t best = kNever;
+            unsigned best_n = 0;
+            for (const auto &kv : shared)
+                if (kv.second > best_n || (kv.second == best_n && kv.first < best))
+                    best = kv.first, best_n = kv.second;
+            if (best == kNever)
+                break;
+            run[nr++] = best;
+            assigned[best] = 1;
+        }
+        for (unsigned i = 0; i < kAsmGroups; ++i)
+            runs.push_back(i < nr ? run[i] : kAsmNoGroup);
+    }
+    const unsigned n_runs = (unsigned)(runs.size() / kAsmGroups);
+    run_grp = runs;
     run_off.assign(n_runs + 1, 0);
     prog.clear();
-    prog.reserve(2 * ((size_t)offsets[n_surfaces] + n_surfaces));
-    constexpr uint32_t kNever = 0xFFFFFFFFu;
-    std::vector<uint32_t> line_of, next_use;
+    prog.reserve(2 * ((size_t)offsets[n_surfaces] + n_surfaces + n_groups));
+    std::vector<uint32_t> ent, next_use; // the run's entries (index into `entries`, or kNever: empty surface)
     std::unordered_map<uint32_t, uint32_t> last_seen;
     for (unsigned r = 0; r < n_runs; ++r)
     {
-        const unsigned s0 = r * kAsmSurfaces, s1 = std::min(n_surfaces, s0 + kAsmSurfaces);
-        const uint32_t k0 = offsets[s0], k1 = offsets[s1];
-        const uint32_t n = k1 - k0;
-        line_of.resize(n);
+        ent.clear();
+        for (unsigned i = 0; i < kAsmGroups; ++i)
+        {
+            const uint32_t g = runs[r * kAsmGroups + i];
+            if (g == kAsmNoGroup)
+                break;
+            for (unsigned s = g * kAsmSurfaces; s < std::min(n_surfaces, (g + 1) * kAsmSurfaces); ++s)
+                for (uint32_t k = offsets[s]; k < offsets[s + 1]; ++k)
+                    ent.push_back(k);
+        }
+        // farthest-next-use slot choice over the run's entries (Belady)
+        const uint32_t n = (uint32_t)ent.size();
         next_use.assign(n, kNever);
         last_seen.clear();
-        for (uint32_t i = 0; i < n; ++i)
-            line_of[i] = entries[k0 + i] & kInvalidId;
         for (uint32_t i = n; i-- > 0;)
         {
-            auto it = last_seen.find(line_of[i]);
+            const uint32_t l = entries[ent[i]] & kInvalidId;
+            auto it = last_seen.find(l);
             next_use[i] = it == last_seen.end() ? kNever : it->second;
-            last_seen[line_of[i]] = i;
+            last_seen[l] = i;
         }
         uint32_t slot_line[kAsmSlots], slot_next[kAsmSlots];
         for (unsigned q = 0; q < kAsmSlots; ++q)
             slot_line[q] = kNever, slot_next[q] = kNever;
-        for (unsigned s = s0; s < s1; ++s)
+        uint32_t i = 0;
+        for (unsigned gi = 0; gi < kAsmGroups; ++gi)
         {
-            if (offsets[s + 1] == offsets[s])
+            const uint32_t g = runs[r * kAsmGroups + gi];
+            if (g == kAsmNoGroup)
+                break;
+            const size_t group_start = prog.size();
+            for (unsigned s = g * kAsmSurfaces; s < std::min(n_surfaces, (g + 1) * kAsmSurfaces); ++s)
             {
-                prog.push_back(0);
-                prog.push_back(kAsmNone | kAsmEnd);
-                continue;
+                if (offsets[s + 1] == offsets[s])
+                {
+                    prog.push_back(0);
+                    prog.push_back(kAsmNone | kAsmEnd);
+                    continue;
+                }
+                for (uint32_t k = offsets[s]; k < offsets[s + 1]; ++k, ++i)
+                {
+                    const uint32_t l = entries[k] & kInvalidId;
+                    uint32_t op = kAsmEval;
+                    int held = -1;
+                    for (unsigned q = 0; q < kAsmSlots; ++q)
+                        if (slot_line[q] == l)
+                            held = (int)q;
+                    if (held >= 0)
+                    {
+                        op = kAsmLoad | ((uint32_t)held << 2);
+                        slot_next[held] = next_use[i];
+                        if (next_use[i] == kNever)
+                            slot_line[held] = kNever;
+                    }
+                    else if (next_use[i] != kNever)
+                    {
+                        // a free slot, else the one whose line is needed last (if later than this one)
+                        int q_use = -1;
+                        for (unsigned q = 0; q < kAsmSlots && q_use < 0; ++q)
+                            if (slot_line[q] == kNever)
+                                q_use = (int)q;
+                        if (q_use < 0)
+                        {
+                            unsigned far = 0;
+                            for (unsigned q = 1; q < kAsmSlots; ++q)
+                                if (slot_next[q] > slot_next[far])
+                                    far = q;
+                            if (slot_next[far] > next_use[i])
+                                q_use = (int)far;
+                        }
+                        if (q_use >= 0)
+                        {
+                            op = kAsmEvalStore | ((uint32_t)q_use << 2);
+                            slot_line[q_use] = l;
+                            slot_next[q_use] = next_use[i];
+                        }
+                    }
+                    if (k + 1 == offsets[s + 1])
+                        op |= kAsmEnd;
+                    prog.push_back(entries[k]);
+                    prog.push_back(op);
+                }
             }
-            for (uint32_t k = offsets[s]; k < offsets[s + 1]; ++k)
+            if (((prog.size() - group_start) / 2) % 2)
             {
-                const uint32_t i = k - k0, l = line_of[i];
-                uint32_t op = kAsmEval;
-                int held = -1;
-                for (unsigned q = 0; q < kAsmSlots; ++q)
-                    if (slot_line[q] == l)
-                        held = (int)q;
-                if (held >= 0)
-                {
-                    op = kAsmLoad | ((uint32_t)held << 2);
-                    slot_next[held] = next_use[i];
-                    if (next_use[i] == kNever)
-                        slot_line[held] = kNever;
-                }
-                else if (next_use[i] != kNever)
-                {
-                    // a free slot, else the one whose line is needed last (if later than this one)
-                    int q_use = -1;
-                    for (unsigned q = 0; q < kAsmSlots && q_use < 0; ++q)
-                        if (slot_line[q] == kNever)
-                            q_use = (int)q;
-                    if (q_use < 0)
-                    {
-                        unsigned far = 0;
-                        for (unsigned q = 1; q < kAsmSlots; ++q)
-                            if (slot_next[q] > slot_next[far])
-                                far = q;
-                        if (slot_next[far] > next_use[i])
-                            q_use = (int)far;
-                    }
-                    if (q_use >= 0)
-                    {
-                        op = kAsmEvalStore | ((uint32_t)q_use << 2);
-                        slot_line[q_use] = l;
-                        slot_next[q_use] = next_use[i];
-                    }
-                }
-                if (k + 1 == offsets[s + 1])
-                    op |= kAsmEnd;
-                prog.push_back(entries[k]);
-                prog.push_back(op);
+                prog.push_back(0); // padding: adds zero, ends no surface
+                prog.push_back(kAsmNone);
             }
         }
         run_off[r + 1] = (uint32_t)(prog.size() / 2);
@@ -369,9 +442,12 @@ int pvl_mesh_create(pvl_mesh **out, unsigned n_points, unsigned n_lines, const u
         st = up(&m->d_line_surfs, lsurf.data(), n_entries);
     if (!st)
     {
-        std::vector<uint32_t> run_off, prog;
-        build_assembly_program(n_surfaces, surface_offsets, surface_lines, run_off, prog);
+        std::vector<uint32_t> run_off, run_grp, prog;
+        build_assembly_program(n_surfaces, surface_offsets, surface_lines, run_off, run_grp, prog);
+        m->n_asm_runs = (unsigned)(run_off.size() - 1);
         st = up(&m->d_asm_off, run_off.data(), run_off.size());
+        if (!st)
+            st = up(&m->d_asm_grp, run_grp.data(), run_grp.size());
         if (!st)
             st = up(&m->d_asm, prog.data(), prog.size());
     }
@@ -396,6 +472,7 @@ int pvl_mesh_destroy(pvl_mesh *m)
     (void)hipFree(m->d_line_offsets);
     (void)hipFree(m->d_line_surfs);
     (void)hipFree(m->d_asm_off);
+    (void)hipFree(m->d_asm_grp);
     (void)hipFree(m->d_asm);
     delete m;
     return PVL_OK;

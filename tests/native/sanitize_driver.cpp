@@ -36,77 +36,111 @@ int orc_lu_factor(unsigned, double *, int *);
 }
 
 // The assembly program builder (pyvl_amd/csrc/topology.hip; operations as in pvl_internal.hpp:
-// kind in bits 0-1 -- 0 evaluate, 1 evaluate and store, 2 load, 3 none --, slot in bits 2-4,
-// bit 8 on a surface's last entry).
+// kind in bits 0-1 -- 0 evaluate, 1 evaluate and store, 2 load, 3 none --, slot in bits 2-5,
+// bit 8 on a surface's last entry; a none entry without bit 8 pads a group to an even length).
 namespace pvl
 {
 void build_assembly_program(unsigned n_surfaces, const uint32_t *offsets, const uint32_t *entries,
-                            std::vector<uint32_t> &run_off, std::vector<uint32_t> &prog);
+                            std::vector<uint32_t> &run_off, std::vector<uint32_t> &run_grp,
+                            std::vector<uint32_t> &prog);
 }
 
-constexpr unsigned kAsmSurfaces = 8; // surfaces per run, as pvl::kAsmSurfaces
+constexpr unsigned kAsmSurfaces = 8; // surfaces per group, as pvl::kAsmSurfaces
+constexpr unsigned kAsmGroups = 4;   // groups per run, as pvl::kAsmGroups
+constexpr unsigned kAsmSlots = 10;   // LDS slots per lane, as pvl::kAsmSlots
 
 // Runs the program as the kernel does, for one control point, with the per-line values
 // v[line] (n . K of that line at the point): slots hold (line, value); a load must find its
-// own line.  Returns the per-surface sums, or an empty vector when the program is malformed.
+// own line; every group starts on an even entry and every surface is produced exactly once.
+// Returns the per-surface sums, or an empty vector when the program is malformed.
 static std::vector<double> run_program(unsigned n_el, const std::vector<uint32_t> &o, const std::vector<uint32_t> &e,
-                                       const std::vector<uint32_t> &run_off, const std::vector<uint32_t> &prog,
-                                       const std::vector<double> &v)
+                                       const std::vector<uint32_t> &run_off, const std::vector<uint32_t> &run_grp,
+                                       const std::vector<uint32_t> &prog, const std::vector<double> &v)
 {
     const unsigned n_runs = (unsigned)run_off.size() - 1;
-    const unsigned per_run = kAsmSurfaces;
-    if (n_runs != (n_el + per_run - 1) / per_run)
+    if (run_grp.size() != (size_t)n_runs * kAsmGroups || run_off[0] != 0)
         return {};
-    std::vector<double> out;
+    std::vector<double> out(n_el, 0.0);
+    std::vector<char> done(n_el, 0);
     for (unsigned r = 0; r < n_runs; ++r)
     {
-        uint32_t slot_line[8];
-        double slot_val[8];
-        for (int q = 0; q < 8; ++q)
+        uint32_t slot_line[kAsmSlots];
+        double slot_val[kAsmSlots];
+        for (unsigned q = 0; q < kAsmSlots; ++q)
             slot_line[q] = 0xFFFFFFFFu, slot_val[q] = 0.0;
         double acc = 0.0;
-        unsigned ends = 0, s = r * per_run;
-        uint32_t k_csr = o[s];
-        for (uint32_t k = run_off[r]; k < run_off[r + 1]; ++k)
+        uint32_t k = run_off[r];
+        for (unsigned gi = 0; gi < kAsmGroups; ++gi)
         {
-            const uint32_t word = prog[2 * k], op = prog[2 * k + 1];
-            const uint32_t kind = op & 3u, slot = (op >> 2) & 7u, line = word & 0x7FFFFFFFu;
-            double val = 0.0;
-            if (kind == 3)
+            const uint32_t g = run_grp[r * kAsmGroups + gi];
+            if (g == 0xFFFFFFFFu)
+                break;
+            if ((k - run_off[r]) % 2 != 0 || (size_t)g * kAsmSurfaces >= n_el)
+                return {};
+            const unsigned s1 = std::min(n_el, (g + 1) * kAsmSurfaces);
+            for (unsigned s = g * kAsmSurfaces; s < s1; ++s)
             {
-                if (o[s + ends] != o[s + ends + 1])
+                if (done[s])
                     return {};
-            }
-            else
-            {
-                if (word != e[k_csr++])
-                    return {};
-                if (kind == 2)
+                uint32_t k_csr = o[s];
+                for (;;)
                 {
-                    if (slot_line[slot] != line)
+                    if (k >= run_off[r + 1])
                         return {};
-                    val = slot_val[slot];
-                }
-                else
-                {
-                    val = v[line];
-                    if (kind == 1)
-                        slot_line[slot] = line, slot_val[slot] = val;
+                    const uint32_t word = prog[2 * k], op = prog[2 * k + 1];
+                    ++k;
+                    const uint32_t kind = op & 3u, slot = (op >> 2) & 15u, line = word & 0x7FFFFFFFu;
+                    if (kind != 3 && slot >= kAsmSlots && kind != 0)
+                        return {};
+                    double val = 0.0;
+                    if (kind == 3)
+                    {
+                        if ((op & (1u << 8)) && o[s] != o[s + 1])
+                            return {};
+                    }
+                    else
+                    {
+                        if (k_csr >= o[s + 1] || word != e[k_csr++])
+                            return {};
+                        if (kind == 2)
+                        {
+                            if (slot_line[slot] != line)
+                                return {};
+                            val = slot_val[slot];
+                        }
+                        else
+                        {
+                            val = v[line];
+                            if (kind == 1)
+                                slot_line[slot] = line, slot_val[slot] = val;
+                        }
+                    }
+                    acc = (word & 0x80000000u) ? acc - val : acc + val;
+                    if (op & (1u << 8))
+                    {
+                        if (k_csr != o[s + 1])
+                            return {};
+                        out[s] = acc;
+                        done[s] = 1;
+                        acc = 0.0;
+                        break;
+                    }
                 }
             }
-            acc = (word & 0x80000000u) ? acc - val : acc + val;
-            if (op & (1u << 8))
+            // a padding entry (none, no end) completes an odd group
+            if ((k - run_off[r]) % 2 != 0)
             {
-                out.push_back(acc);
-                acc = 0.0;
-                ++ends;
-                if (s + ends < n_el && k_csr != o[s + ends])
+                if (k >= run_off[r + 1] || prog[2 * k + 1] != 3u)
                     return {};
+                ++k;
             }
         }
-        if (ends != std::min(per_run, n_el - r * per_run))
+        if (k != run_off[r + 1])
             return {};
     }
+    for (unsigned s = 0; s < n_el; ++s)
+        if (!done[s])
+            return {};
     return out;
 }
 
@@ -199,8 +233,8 @@ int main()
         orc_line_forces(nl1, l1.data(), dl1.data(), circ.data(), pos.data(), vel.data(), lf.data());
         // the assembly program reproduces every surface's sum for every control point
         {
-            std::vector<uint32_t> run_off, prog;
-            pvl::build_assembly_program(n_el, o1.data(), e1.data(), run_off, prog);
+            std::vector<uint32_t> run_off, run_grp, prog;
+            pvl::build_assembly_program(n_el, o1.data(), e1.data(), run_off, run_grp, prog);
             for (unsigned c = 0; c < n_cpts && !bad; ++c)
             {
                 std::vector<double> v(nl1 ? nl1 : 1);
@@ -209,7 +243,7 @@ int main()
                     const double *kv = &lk[3 * ((size_t)c * nl1 + l)];
                     v[l] = kv[0] * nrm[3 * c] + kv[1] * nrm[3 * c + 1] + kv[2] * nrm[3 * c + 2];
                 }
-                const std::vector<double> got = run_program(n_el, o1, e1, run_off, prog, v);
+                const std::vector<double> got = run_program(n_el, o1, e1, run_off, run_grp, prog, v);
                 if (got.size() != n_el)
                 {
                     bad |= fail("assembly program structure", k);
