@@ -2018,7 +2018,7 @@ template <typename T> int factor_impl(int n, T *A, int *piv, int *info_out, bool
         st2[i] = g_status_host[i];
     if (st2[1])
     {
-        set_error("LU panel barrier timed out");
+        set_error("LU panel exchange timed out waiting for another workgroup's record");
         return PVL_ERR_HIP;
     }
     *info_out = pivoting ? st2[0] : (st2[2] == 0x7F7F7F7F ? 0 : st2[2]);
