@@ -1238,8 +1238,13 @@ __device__ __forceinline__ void flow_substitute(T (&x)[QW], const T (*Ls)[BS + 1
 #pragma unroll
             for (int j = 0; j < NQ; ++j)
             {
-                if (!LOWER && lane == k)
-                    x[j] = x[j] / diag;
+                if (!LOWER)
+                {
+                    // every lane divides and the pivot lane keeps its quotient: no branch, so the
+                    // right-hand sides' divisions interleave
+                    const T q = x[j] / diag;
+                    x[j] = lane == k ? q : x[j];
+                }
                 const T xk = bcast_lane(x[j], k);
                 if (LOWER ? lane > k : lane < k)
                     x[j] = fma(-lc[u], xk, x[j]);
