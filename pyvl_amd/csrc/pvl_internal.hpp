@@ -33,7 +33,7 @@ std::recursive_mutex &api_mutex();
 // has been synchronised, PVL_OK otherwise.
 int take_async_error();
 
-// Launch of a kernel whose workgroups wait for each other (grid barriers), with grid size
+// Launch of a kernel whose workgroups wait for each other (grid barriers, exchange rounds), with grid size
 // checked against what the device holds at once.  By default a cooperative launch; with
 // PVL_COOPERATIVE=0 in the environment an ordinary launch of the same grid (every
 // grid-wide wait in the library is bounded and reports an error rather than hanging), for
