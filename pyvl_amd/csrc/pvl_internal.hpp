@@ -34,10 +34,9 @@ std::recursive_mutex &api_mutex();
 int take_async_error();
 
 // Launch of a kernel whose workgroups wait for each other (grid barriers, exchange rounds), with grid size
-// checked against what the device holds at once.  By default a cooperative launch; with
-// PVL_COOPERATIVE=0 in the environment an ordinary launch of the same grid (every
-// grid-wide wait in the library is bounded and reports an error rather than hanging), for
-// tools that cannot follow cooperative launches.
+// checked against what the device holds at once.  By default an ordinary launch (every
+// grid-wide wait in the library is bounded and reports an error rather than hanging); with
+// PVL_COOPERATIVE=1 in the environment a cooperative launch of the same grid.
 int launch_grid_synced(const void *func, unsigned grid, unsigned block, void **args, size_t lds, hipStream_t s);
 
 // Release of the process-lifetime workspaces (pvl_shutdown).

@@ -113,13 +113,19 @@ int side_groups(int per_cu)
     return per_cu * cus;
 }
 
+// Ordinary launches by default: every grid-synced kernel runs on the library stream (never
+// two at once), no other kernel waits on one of them from the device, and each grid fits on
+// the otherwise idle device (checked below), so every group becomes resident once the CUs
+// that other streams' kernels hold are free; every wait on another group is bounded.  A
+// cooperative launch (PVL_COOPERATIVE=1) adds ~25 us per launch, 6 ms over the 256 panels of
+// a 16k factorisation.
 static bool cooperative_launches()
 {
     static int on = -1;
     if (on < 0)
     {
         const char *e = getenv("PVL_COOPERATIVE");
-        on = !(e && e[0] == '0');
+        on = e && e[0] == '1';
     }
     return on != 0;
 }
