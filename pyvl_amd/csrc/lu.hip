@@ -13,10 +13,10 @@
 //   laswp_kernel   applies the panel's interchanges to the other columns.
 //   trsm_kernel    U12 = L11^-1 A12 (one lane per column in registers, L11 in LDS).
 //   gemm_kernel    A22 -= L21 U12 on MFMA (v_mfma_f64_16x16x4f64 / v_mfma_f32_16x16x4f32)
-//                  for the updates inside an outer block and for n <= 2048; the trailing
-//                  update after each outer block of 256 columns is a plain GEMM and goes to
-//                  rocBLAS (dgemm / sgemm, no atomics), which runs it ~1.4x faster at these
-//                  shapes (m x m x 256).
+//                  for n <= 2048 and the short updates of larger systems; the trailing
+//                  update after each outer block of 256 columns, and the updates inside an
+//                  outer block with a side of 2048 or more, are plain GEMMs and go to rocBLAS
+//                  (dgemm / sgemm, no atomics), which runs m x m x 256 ~1.4x faster.
 // Solves: one workgroup for systems whose right-hand sides fit its LDS (trsv_wg_kernel),
 // else one launch per triangle with a workgroup per block row that waits on the flags of
 // the blocks before it (trsv_flow_kernel).
@@ -1955,8 +1955,9 @@ template <typename T> int factor_impl(int n, T *A, int *piv, int *info_out, bool
         const int nt = (n2 + 63) / 64, cpx = (nt + kXcds - 1) / kXcds;
         const int mt = (m2 + 63) / 64;
         ProfScope prof(sizeof(T) == 8 ? "lu_gemm_f64" : "lu_gemm_f32", 2.0 * m2 * n2 * K);
-        if (K > NB)
-            // an outer block's trailing update: one sum over the block's K columns
+        if (K > NB || (nbo > nbf && std::max(m2, n2) >= 2048))
+            // an outer block's trailing update (one sum over the block's K columns), and for
+            // n > 2048 the long updates inside an outer block
             return blas_update<T>(A + (size_t)row0 * n + col0, A + (size_t)row0 * n + kk0, A + (size_t)kk0 * n + col0, n,
                                   m2, n2, K);
         bool vec = false;
