@@ -354,3 +354,46 @@ def test_residual_matches_numpy(n, nrhs):
     want = b - a @ x
     # each entry is a sum of n products of size ~1: allow a few ulps of that scale
     assert np.max(np.abs(r.reshape(want.shape) - want)) <= 1e-13 * np.sqrt(n) * 10
+
+
+_COOP_CHILD = """
+import sys
+import numpy as np
+from pyvl_amd import linalg as LA
+n, out = int(sys.argv[1]), sys.argv[2]
+a = np.random.default_rng(n).normal(size=(n, n))
+r = {}
+for name, prec in (("fp64", LA.FP64), ("fp32", LA.FP32)):
+    h = LA.LU(n, prec).factor(a)
+    r[name + "_lu"], r[name + "_piv"] = h.factors(), h.pivots()
+np.savez(out, **r)
+print("child done", flush=True)
+"""
+
+
+def test_cooperative_launch_option_gives_identical_factors(tmp_path):
+    """The grid-wide panel is an ordinary launch by default; PVL_COOPERATIVE=1 (read once
+    per process, so set in a child process) launches it cooperatively.  n = 2,600 takes the
+    64-column grid panel for every panel and the rocBLAS outer updates: factors and pivots
+    of both launch kinds must be equal bit for bit, in FP64 and FP32."""
+    import os
+    import subprocess
+    import sys
+    from pathlib import Path
+
+    n = 2600
+    out = tmp_path / "coop.npz"
+    root = Path(__file__).resolve().parents[1]
+    env = dict(os.environ, PVL_COOPERATIVE="1")
+    print(f"cooperative-launch child: n={n}", flush=True)
+    p = subprocess.run([sys.executable, "-c", _COOP_CHILD, str(n), str(out)], cwd=root, env=env,
+                       capture_output=True, text=True, timeout=150)
+    assert p.returncode == 0, p.stdout[-2000:] + p.stderr[-2000:]
+    got = np.load(out)
+    a = np.random.default_rng(n).normal(size=(n, n))
+    for name, prec in (("fp64", LA.FP64), ("fp32", LA.FP32)):
+        h = LA.LU(n, prec).factor(a)
+        assert np.array_equal(got[name + "_piv"], h.pivots()), name
+        assert np.array_equal(got[name + "_lu"], h.factors()), name
+    lu_ref, piv_ref = sla.lu_factor(a)
+    assert np.array_equal(got["fp64_piv"], piv_ref)
