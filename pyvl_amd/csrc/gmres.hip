@@ -4,7 +4,7 @@
 // warm-starts the refinement, and a solve that does not converge reports so the caller can
 // fall back to LU.
 //
-// One GMRES(m) cycle is a single cooperative kernel: workgroup g owns a block of rows;
+// One GMRES(m) cycle is a single grid-wide kernel: workgroup g owns a block of rows;
 // per Arnoldi step it forms w = A M^-1 v_j for its rows (M = diag(A), right Jacobi
 // preconditioning), orthogonalises with classical Gram-Schmidt applied twice (partial dot
 // products of all groups combined after a grid barrier), normalises, and every group
@@ -25,7 +25,7 @@ namespace
 {
 
 constexpr int kMaxM = 48;          // restart length cap
-constexpr int kGroups = 128;       // cooperative grid size cap
+constexpr int kGroups = 128;       // grid size cap (every group resident at once)
 constexpr unsigned kSpin = 20000000u;
 
 __device__ __forceinline__ void gbar(unsigned *counter, unsigned target, int *err)

@@ -7,7 +7,7 @@
 // same matrix, with piv[j] = row swapped with row j at step j (0-based, as scipy returns).
 //
 // Right-looking blocked algorithm, panel width NB = 64:
-//   panel_kernel   cooperative grid; each workgroup holds up to R panel rows in LDS; per
+//   panel_kernel   grid-wide; each workgroup holds up to R panel rows in LDS; per
 //                  column every group publishes its pivot candidate's row and then a tagged
 //                  record of the candidate, and reads every group's record (no barrier).
 //   laswp_kernel   applies the panel's interchanges to the other columns.
