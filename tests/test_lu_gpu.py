@@ -397,3 +397,24 @@ def test_cooperative_launch_option_gives_identical_factors(tmp_path):
         assert np.array_equal(got[name + "_lu"], h.factors()), name
     lu_ref, piv_ref = sla.lu_factor(a)
     assert np.array_equal(got["fp64_piv"], piv_ref)
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_repeated_factorisation_is_bit_identical(prec):
+    """n = 3,000 takes the rocBLAS updates (atomics off, no split-K): two factorisations of
+    the same matrix, with another one of a different matrix in between, give the same
+    factors and pivots bit for bit."""
+    n = 3000
+    p = LA.FP64 if prec == "fp64" else LA.FP32
+    rng = np.random.default_rng(11)
+    a, other = rng.normal(size=(n, n)), rng.normal(size=(n, n))
+    h = LA.LU(n, p)
+    lu1, piv1 = h.factor(a).factors(), h.pivots()
+    h.factor(other)
+    lu2, piv2 = h.factor(a).factors(), h.pivots()
+    assert np.array_equal(piv1, piv2)
+    assert np.array_equal(lu1, lu2)
+    if prec == "fp64":
+        lu_ref, piv_ref = sla.lu_factor(a)
+        assert np.array_equal(piv1, piv_ref)
+        assert _rel(lu1, lu_ref) < 1e-12
