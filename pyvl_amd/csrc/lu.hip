@@ -845,18 +845,21 @@ __global__ void __launch_bounds__(256) laswp_kernel(T *__restrict__ A, int lda, 
 // lanes per column: lane p of a quad holds rows p, p + 4, .. of its column in registers.  Step
 // t broadcasts the final x_t from its owner lane over the quad (DPP) and every lane subtracts
 // l_it x_t from its rows i > t, with the l_it each lane needs contiguous in LDS
-// (Lq[t][p * KQ + k] = l_(4k+p),t).  Per element the operations are those of dtrsm's column
-// sweep.  No barrier after the load.  JB = panel width (32 or 64).
+// (Lq[t][p * KP + k] = l_(4k+p),t).  The four segments of a row start KP = KQ + 2 elements
+// apart, so the four addresses a step's read takes fall on distinct LDS banks (KQ apart they
+// all fell on one), and the row stride is odd, so the transposed fill spreads its stores over
+// the banks.  Per element the operations are those of dtrsm's column sweep.  No barrier after
+// the load.  JB = panel width (32 or 64).
 template <typename T, int JB>
 __global__ void __launch_bounds__(256) trsm_kernel(T *__restrict__ A, int lda, int c0, int c1, int k)
 {
     main_stream_priority();
-    constexpr int KQ = JB / 4;
-    __shared__ T Lq[JB][JB];
+    constexpr int KQ = JB / 4, KP = KQ + 2, RS = 4 * KP + 1;
+    __shared__ T Lq[JB][RS];
     for (int idx = threadIdx.x; idx < JB * JB; idx += 256)
     {
         const int i = idx / JB, t = idx % JB;
-        Lq[t][(i & 3) * KQ + (i >> 2)] = t < i ? A[(size_t)(k + i) * lda + k + t] : T(0);
+        Lq[t][(i & 3) * KP + (i >> 2)] = t < i ? A[(size_t)(k + i) * lda + k + t] : T(0);
     }
     __syncthreads();
     const int lane = threadIdx.x & 63, p = lane & 3;
@@ -880,7 +883,7 @@ __global__ void __launch_bounds__(256) trsm_kernel(T *__restrict__ A, int lda, i
         case 2: xt = quad_bcast<2>(x[kt]); break;
         default: xt = quad_bcast<3>(x[kt]); break;
         }
-        const T *lrow = &Lq[t][p * KQ];
+        const T *lrow = &Lq[t][p * KP];
         {
             const T v = fma(-lrow[kt], xt, x[kt]);
             if (p > pt)
