@@ -1229,6 +1229,12 @@ __device__ __forceinline__ void flow_substitute(T (&x)[QW], const T (*Ls)[BS + 1
 {
     constexpr int KC = 8;
     T lc[KC], ln[KC];
+    // upper: the lane's own reciprocal 1/u_kk, correctly rounded (a true division), ahead of
+    // the chain; a step's quotient is then q0 = x r, e = x - u q0 (exact with the FMA),
+    // q = q0 + e r: the correctly rounded x / u_kk (Markstein's correction with a correctly
+    // rounded reciprocal; exact wherever e does not underflow, |x| >= 2^-969, else within an
+    // ulp) in three dependent operations instead of a division sequence
+    const T rdiag = LOWER ? T(0) : T(1) / diag;
 #pragma unroll
     for (int u = 0; u < KC; ++u)
         lc[u] = Ls[lane][LOWER ? u : BS - 1 - u];
@@ -1249,9 +1255,11 @@ __device__ __forceinline__ void flow_substitute(T (&x)[QW], const T (*Ls)[BS + 1
             {
                 if (!LOWER)
                 {
-                    // every lane divides and the pivot lane keeps its quotient: no branch, so the
-                    // right-hand sides' divisions interleave
-                    const T q = x[j] / diag;
+                    // every lane forms the quotient and the pivot lane keeps it: no branch, so
+                    // the right-hand sides' quotients interleave
+                    const T q0 = x[j] * rdiag;
+                    const T e = fma(-q0, diag, x[j]);
+                    const T q = fma(e, rdiag, q0);
                     x[j] = lane == k ? q : x[j];
                 }
                 const T xk = bcast_lane(x[j], k);
