@@ -1900,16 +1900,29 @@ template <typename T> int panel_rows_max() { return 256; }
         }                                                                                                             \
     } while (0)
 
-// The handle bound to the library stream.
+// Configuration of a new handle: deterministic results (no split-K atomics), full-precision
+// FP32 products, scalars from the host.
+int blas_configure(rocblas_handle h)
+{
+    PVL_BLAS(rocblas_set_atomics_mode(h, rocblas_atomics_not_allowed));
+    PVL_BLAS(rocblas_set_math_mode(h, rocblas_default_math));
+    PVL_BLAS(rocblas_set_pointer_mode(h, rocblas_pointer_mode_host));
+    return PVL_OK;
+}
+
+// The handle bound to the library stream; kept only once fully configured.
 int blas_handle()
 {
     if (!g_blas)
     {
-        PVL_BLAS(rocblas_create_handle(&g_blas));
-        // deterministic results (no split-K atomics) and full-precision FP32 products
-        PVL_BLAS(rocblas_set_atomics_mode(g_blas, rocblas_atomics_not_allowed));
-        PVL_BLAS(rocblas_set_math_mode(g_blas, rocblas_default_math));
-        PVL_BLAS(rocblas_set_pointer_mode(g_blas, rocblas_pointer_mode_host));
+        rocblas_handle h = nullptr;
+        PVL_BLAS(rocblas_create_handle(&h));
+        if (int st = blas_configure(h))
+        {
+            (void)rocblas_destroy_handle(h);
+            return st;
+        }
+        g_blas = h;
     }
     PVL_BLAS(rocblas_set_stream(g_blas, stream()));
     return PVL_OK;
