@@ -854,23 +854,33 @@ template <typename T, int JB>
 __global__ void __launch_bounds__(256) trsm_kernel(T *__restrict__ A, int lda, int c0, int c1, int k)
 {
     main_stream_priority();
-    constexpr int KQ = JB / 4, KP = KQ + 2, RS = 4 * KP + 1;
+    constexpr int KQ = JB / 4, KP = KQ + 2, RS = 4 * KP + 1, NL = JB * JB / 256;
     __shared__ T Lq[JB][RS];
-    for (int idx = threadIdx.x; idx < JB * JB; idx += 256)
-    {
-        const int i = idx / JB, t = idx % JB;
-        Lq[t][(i & 3) * KP + (i >> 2)] = t < i ? A[(size_t)(k + i) * lda + k + t] : T(0);
-    }
-    __syncthreads();
     const int lane = threadIdx.x & 63, p = lane & 3;
-    const int col = c0 + (blockIdx.x * 4 + (threadIdx.x >> 6)) * 16 + (lane >> 2);
-    if (c0 + (blockIdx.x * 4 + (threadIdx.x >> 6)) * 16 >= c1)
-        return;
+    const int wcol0 = c0 + (blockIdx.x * 4 + (threadIdx.x >> 6)) * 16;
+    const int col = wcol0 + (lane >> 2);
     const bool live = col < c1;
-    T x[KQ];
+    // the column's rows and L11 are loaded together (all loads in flight before the first is
+    // used; filling LDS load by load had cost two dependent memory latencies more)
+    T x[KQ], lv[NL];
 #pragma unroll
     for (int q = 0; q < KQ; ++q)
         x[q] = live ? A[(size_t)(k + 4 * q + p) * lda + col] : T(0);
+#pragma unroll
+    for (int r = 0; r < NL; ++r)
+    {
+        const int idx = threadIdx.x + 256 * r, i = idx / JB, t = idx % JB;
+        lv[r] = t < i ? A[(size_t)(k + i) * lda + k + t] : T(0);
+    }
+#pragma unroll
+    for (int r = 0; r < NL; ++r)
+    {
+        const int idx = threadIdx.x + 256 * r, i = idx / JB, t = idx % JB;
+        Lq[t][(i & 3) * KP + (i >> 2)] = lv[r];
+    }
+    __syncthreads();
+    if (wcol0 >= c1)
+        return;
 #pragma unroll
     for (int t = 0; t < JB - 1; ++t)
     {
